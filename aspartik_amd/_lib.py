@@ -1,0 +1,108 @@
+"""Loader of the in-tree HIP library ``libb3hip.so`` (C ABI: include/b3hip.h).
+
+There is no CPU fallback: if the library is missing or cannot be loaded,
+:func:`lib` raises.  Build it with ``__graft_entry__.build()`` or
+``make -C aspartik_amd/csrc``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libb3hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "b3hip.h")
+
+B3H_OK = 0
+STATUS_NAMES = {
+    0: "B3H_OK",
+    1: "B3H_ERR_ARG",
+    2: "B3H_ERR_NO_DEVICE",
+    3: "B3H_ERR_OOM",
+    4: "B3H_ERR_HIP",
+    5: "B3H_ERR_STATE",
+    6: "B3H_ERR_UNSUPPORTED",
+}
+
+B3H_F32_SCALED = 0
+B3H_F64_PLAIN = 1
+B3H_F16_MFMA = 2
+B3H_MODEL_SPECIAL_CASES = 1
+
+
+class B3hError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("precision", C.c_uint32),
+        ("states", C.c_uint32),
+        ("scale_ln", C.c_uint32),
+        ("tile_patterns", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
+    ]
+
+
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+_lib = None
+
+# name -> (restype, argtypes); every symbol declared in include/b3hip.h
+SIGNATURES = {
+    "b3h_config_default": (None, [C.POINTER(Config)]),
+    "b3h_create": (C.c_int, [C.POINTER(Config), u8p, C.c_uint32, C.c_uint32, C.c_uint32,
+                             C.POINTER(C.c_void_p)]),
+    "b3h_destroy": (C.c_int, [C.c_void_p]),
+    "b3h_set_weights": (C.c_int, [C.c_void_p, u32p]),
+    "b3h_propose": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, f64p, f64p]),
+    "b3h_set_model": (C.c_int, [C.c_void_p, f64p, f64p, C.c_uint32]),
+    "b3h_propose_times": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, f64p, f64p]),
+    "b3h_accept": (C.c_int, [C.c_void_p]),
+    "b3h_reject": (C.c_int, [C.c_void_p]),
+    "b3h_likelihoods": (C.c_int, [C.c_void_p, C.c_uint32, f64p]),
+    "b3h_site_mixed": (C.c_int, [C.c_void_p, f64p]),
+    "b3h_total": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "b3h_num_patterns": (C.c_uint32, [C.c_void_p]),
+    "b3h_last_error": (C.c_char_p, []),
+    "b3h_device_count": (C.c_int, []),
+    "b3h_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "b3h_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "b3h_device_bytes": (C.c_uint64, [C.c_void_p]),
+    "b3h_gamma_rates": (C.c_int, [C.c_double, C.c_uint32, f64p]),
+}
+
+
+def lib():
+    """Load libb3hip.so; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() "
+            "(the HIP library is required: there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != B3H_OK:
+        msg = lib().b3h_last_error()
+        raise B3hError(status, msg.decode() if msg else "")
+
+
+def device_count() -> int:
+    return int(lib().b3h_device_count())

@@ -1,0 +1,152 @@
+"""Calculator configuration and the HIP calculator handle.
+
+Mirrors b3's ``Calculator`` configuration object (crates/b3/src/calculator/
+mod.rs:94-156: ``Calculator.CPU()``, ``.CUDAPartials()``, ``.with_scale()``)
+with MI355X backends, and the ``Calculator<f64>`` trait (mod.rs:54-76)
+through the C ABI of libb3hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .. import _lib
+from .._lib import check, lib
+
+
+class Calculator:
+    """Backend choice + scaling, like b3's ``CalculatorConfig``."""
+
+    def __init__(self, precision: int, device: int = 0, tile_patterns: int = 0,
+                 scale_ln: int = 30, states: int = 4):
+        self.precision = int(precision)
+        self.device = int(device)
+        self.tile_patterns = int(tile_patterns)
+        self.scale_ln = int(scale_ln)
+        self.states = int(states)
+
+    @classmethod
+    def HIP(cls, device: int = 0, tile_patterns: int = 0):
+        """FP32 partials with exact power-of-two scaling (the product path)."""
+        return cls(_lib.B3H_F32_SCALED, device, tile_patterns)
+
+    @classmethod
+    def HIPPlainF64(cls, device: int = 0):
+        """FP64 plain pruning: b3's GPU algorithm at FP64 (the baseline)."""
+        return cls(_lib.B3H_F64_PLAIN, device)
+
+    def with_scale(self, scale_ln: int):
+        """calculator/mod.rs:149-155"""
+        self.scale_ln = int(scale_ln)
+        return self
+
+    def make(self, samples, num_patterns: int, num_categories: int = 1):
+        """CalculatorConfig::make4 (calculator/mod.rs:159-200) for K
+        categories in one handle."""
+        return HipCalculator(self, samples, num_patterns, num_categories)
+
+    def __repr__(self):
+        names = {0: "HIP", 1: "HIPPlainF64", 2: "HIPCodon"}
+        return (f"Calculator.{names.get(self.precision, self.precision)}(device={self.device}, "
+                f"scale_ln={self.scale_ln})")
+
+
+class HipCalculator:
+    """One libb3hip handle: K rate categories of one alignment."""
+
+    def __init__(self, config: Calculator, samples, num_patterns: int, num_categories: int):
+        samples = np.ascontiguousarray(samples, np.uint8).reshape(-1)
+        self.P = int(num_patterns)
+        if samples.size % self.P:
+            raise ValueError("samples size is not a multiple of num_patterns")
+        self.L = samples.size // self.P
+        self.K = int(num_categories)
+        cfg = _lib.Config()
+        lib().b3h_config_default(C.byref(cfg))
+        cfg.device = config.device
+        cfg.precision = config.precision
+        cfg.states = config.states
+        cfg.scale_ln = config.scale_ln
+        cfg.tile_patterns = config.tile_patterns
+        h = C.c_void_p()
+        check(lib().b3h_create(C.byref(cfg), samples, self.L, self.P, self.K, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().b3h_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- Calculator<f64> -------------------------------------------------
+    def propose(self, internals, children, matrices, frequencies):
+        internals = np.ascontiguousarray(internals, np.uint32)
+        check(lib().b3h_propose(self._h, internals.size, internals,
+                                np.ascontiguousarray(children, np.uint32).reshape(-1),
+                                np.ascontiguousarray(matrices, np.float64).reshape(-1),
+                                np.ascontiguousarray(frequencies, np.float64)))
+
+    def propose_times(self, internals, children, edge_times, category_rates):
+        internals = np.ascontiguousarray(internals, np.uint32)
+        check(lib().b3h_propose_times(self._h, internals.size, internals,
+                                      np.ascontiguousarray(children, np.uint32).reshape(-1),
+                                      np.ascontiguousarray(edge_times, np.float64),
+                                      np.ascontiguousarray(category_rates, np.float64)))
+
+    def set_model(self, q, frequencies, special_cases: bool = True):
+        flags = _lib.B3H_MODEL_SPECIAL_CASES if special_cases else 0
+        check(lib().b3h_set_model(self._h, np.ascontiguousarray(q, np.float64).reshape(-1),
+                                  np.ascontiguousarray(frequencies, np.float64), flags))
+
+    def accept(self):
+        check(lib().b3h_accept(self._h))
+
+    def reject(self):
+        check(lib().b3h_reject(self._h))
+
+    def likelihoods(self, category: int = 0):
+        out = np.empty(self.P, np.float64)
+        check(lib().b3h_likelihoods(self._h, int(category), out))
+        return out
+
+    def site_mixed(self):
+        out = np.empty(self.P, np.float64)
+        check(lib().b3h_site_mixed(self._h, out))
+        return out
+
+    def total(self) -> float:
+        v = C.c_double()
+        check(lib().b3h_total(self._h, C.byref(v)))
+        return v.value
+
+    def num_patterns(self) -> int:
+        return int(lib().b3h_num_patterns(self._h))
+
+    def set_weights(self, weights):
+        check(lib().b3h_set_weights(self._h, np.ascontiguousarray(weights, np.uint32)))
+
+    # ---- measurement -----------------------------------------------------
+    def profile_enable(self, on: bool = True):
+        check(lib().b3h_profile_enable(self._h, int(bool(on))))
+
+    def profile_read(self):
+        ms = C.c_double()
+        n = C.c_uint64()
+        check(lib().b3h_profile_read(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def device_bytes(self) -> int:
+        return int(lib().b3h_device_bytes(self._h))
+
+
+def gamma_rates(alpha: float, num_categories: int):
+    """Discrete-gamma category rates (likelihood/gamma.rs:205-225)."""
+    out = np.empty(int(num_categories), np.float64)
+    check(lib().b3h_gamma_rates(float(alpha), int(num_categories), out))
+    return out

@@ -1,0 +1,626 @@
+// Host side of the b3hip C ABI (include/b3hip.h): handle lifecycle, the
+// EditBuf slot bookkeeping, the update-list schedule and kernel launches.
+#include "../../include/b3hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "schedule.hpp"
+
+using namespace b3h;
+
+namespace {
+
+thread_local std::string g_err;
+
+b3h_status fail(b3h_status s, const std::string &msg) {
+  g_err = msg;
+  return s;
+}
+
+#define HIPCHK(expr)                                                   \
+  do {                                                                 \
+    hipError_t e_ = (expr);                                            \
+    if (e_ != hipSuccess)                                              \
+      return fail(B3H_ERR_HIP, std::string(#expr) + ": " +             \
+                                   hipGetErrorString(e_));             \
+  } while (0)
+
+constexpr int kStages = 2;
+constexpr uint32_t kMaxSlots = 32;
+constexpr size_t kMaxLds = 160 * 1024;
+
+}  // namespace
+
+struct b3h_calc {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t precision = B3H_F32_SCALED;
+  uint32_t L = 0, I = 0, P = 0, K = 0, S = 4;
+  uint32_t scale_ln = 30;
+  uint32_t T = 1;  // 64-pattern sub-tiles per workgroup
+  uint32_t blocks = 0;
+
+  // device buffers
+  uint8_t *d_codes = nullptr;
+  void *d_partials = nullptr;   // float4 or double4 [2I][K][P]
+  int16_t *d_exps = nullptr;    // F32 path
+  uint8_t *d_scales = nullptr;  // F64 path flags [2I][K][P]
+  uint32_t *d_scale_sums = nullptr, *d_scale_sums_bak = nullptr;
+  double *d_site = nullptr, *d_mixed = nullptr;
+  uint32_t *d_weights = nullptr;
+  double *d_blocksum = nullptr, *d_mix_blocksum = nullptr;
+  uint32_t *d_counter = nullptr;
+  double *d_total = nullptr;
+  void *d_pmats = nullptr;  // [I][2][K][S*S] float or double
+  DevModel *d_model = nullptr;
+  bool model_set = false;
+
+  // staging (pinned host -> device), double buffered
+  size_t stage_bytes = 0;
+  unsigned char *h_stage[kStages] = {nullptr, nullptr};
+  unsigned char *d_stage[kStages] = {nullptr, nullptr};
+  hipEvent_t stage_ev[kStages] = {nullptr, nullptr};
+  int stage_next = 0;
+
+  // host state
+  std::vector<uint8_t> editbuf;  // crates/sk/src/editbuf.rs
+  std::vector<uint8_t> editbuf_next;
+  Scheduler sched;
+  double freqs[64] = {0};
+  bool has_result = false;
+  uint64_t bytes = 0;
+
+  // profiling
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_start, ev_stop;
+  size_t ev_used = 0;
+  double prof_ms = 0.0;
+  uint64_t prof_launches = 0;
+};
+
+namespace {
+
+b3h_status dev_alloc(b3h_calc *c, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(B3H_ERR_OOM, "hipMalloc(" + std::to_string(bytes) +
+                                 " B) failed: " + hipGetErrorString(e));
+  }
+  c->bytes += bytes;
+  return B3H_OK;
+}
+
+void release(b3h_calc *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void *ptrs[] = {c->d_codes, c->d_partials, c->d_exps, c->d_scales,
+                  c->d_scale_sums, c->d_scale_sums_bak, c->d_site, c->d_mixed,
+                  c->d_weights, c->d_blocksum, c->d_mix_blocksum,
+                  c->d_counter, c->d_total, c->d_pmats, c->d_model};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int s = 0; s < kStages; s++) {
+    if (c->h_stage[s]) (void)hipHostFree(c->h_stage[s]);
+    if (c->d_stage[s]) (void)hipFree(c->d_stage[s]);
+    if (c->stage_ev[s]) (void)hipEventDestroy(c->stage_ev[s]);
+  }
+  for (auto e : c->ev_start) (void)hipEventDestroy(e);
+  for (auto e : c->ev_stop) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+// Tip codes as the kernels read them: 4 states use cpu4.rs:117-125's remap
+// {1,2,4,8} -> {0,1,2,3}, everything else 4 (unknown); S > 4 keeps indices,
+// clamping unknowns to S.
+uint8_t remap_code(uint8_t v, uint32_t S) {
+  if (S == 4) {
+    switch (v) {
+      case 1: return 0;
+      case 2: return 1;
+      case 4: return 2;
+      case 8: return 3;
+      default: return 4;
+    }
+  }
+  return v < S ? v : (uint8_t)S;
+}
+
+b3h_status check_handle(const b3h_calc *c) {
+  if (!c) return fail(B3H_ERR_ARG, "null calculator handle");
+  return B3H_OK;
+}
+
+b3h_status set_device(const b3h_calc *c) {
+  HIPCHK(hipSetDevice(c->device));
+  return B3H_OK;
+}
+
+// Validate an update list: node ids, child ids, root last, children before
+// parents within the list.
+b3h_status validate_list(const b3h_calc *c, uint32_t n,
+                         const uint32_t *internals, const uint32_t *children) {
+  if (n == 0) return fail(B3H_ERR_ARG, "empty update list");
+  if (n > c->I) return fail(B3H_ERR_ARG, "update list longer than internal node count");
+  if (!internals || !children) return fail(B3H_ERR_ARG, "null update list");
+  const uint32_t N = 2 * c->L - 1;
+  for (uint32_t i = 0; i < n; i++) {
+    if (internals[i] < c->L || internals[i] >= N)
+      return fail(B3H_ERR_ARG, "internal node id out of range at " + std::to_string(i));
+    for (int s = 0; s < 2; s++)
+      if (children[2 * i + s] >= N || children[2 * i + s] == internals[i])
+        return fail(B3H_ERR_ARG, "child node id out of range at " + std::to_string(i));
+  }
+  return B3H_OK;
+}
+
+unsigned char *next_stage(b3h_calc *c, int *which) {
+  int s = c->stage_next;
+  c->stage_next = (s + 1) % kStages;
+  (void)hipEventSynchronize(c->stage_ev[s]);  // previous copy from it is done
+  *which = s;
+  return c->h_stage[s];
+}
+
+b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
+                       const uint32_t *children, const double *mats,
+                       const double *times, const double *rates,
+                       const double *freqs) {
+  b3h_status st = set_device(c);
+  if (st) return st;
+  st = validate_list(c, n, internals, children);
+  if (st) return st;
+  const uint32_t K = c->K, S = c->S, SS = S * S;
+
+  // EditBuf::set_edited for every node of the list (cpu4.rs:101-106), on a
+  // copy that is committed only once the list has been validated
+  c->editbuf_next = c->editbuf;
+  for (uint32_t i = 0; i < n; i++) {
+    uint8_t &m = c->editbuf_next[internals[i] - c->L];
+    m = (uint8_t)(((m & 1u) ^ (uint8_t)~(m >> 1)) & 3u);
+  }
+
+  const bool f32 = c->precision == B3H_F32_SCALED;
+  const uint32_t cap = f32 ? c->sched.capacity : 0;
+  std::string err;
+  if (!c->sched.build(n, internals, children, c->L, c->editbuf_next.data(),
+                      /*reorder=*/f32, cap, &err))
+    return fail(B3H_ERR_ARG, err);
+  c->editbuf.swap(c->editbuf_next);
+
+  // stage: entries | matrices (mode 0) or times + rates (mode 1)
+  const size_t ebytes = (size_t)n * sizeof(Entry);
+  const size_t mbytes = mats ? (size_t)K * 2 * n * SS * 8 : 0;
+  const size_t tbytes = times ? (size_t)2 * n * 8 : 0;
+  const size_t rbytes = times ? (size_t)K * 8 : 0;
+  const size_t off_m = (ebytes + 255) & ~(size_t)255;
+  const size_t off_t = off_m + ((mbytes + 255) & ~(size_t)255);
+  const size_t off_r = off_t + ((tbytes + 255) & ~(size_t)255);
+  const size_t total = off_r + rbytes;
+  if (total > c->stage_bytes) return fail(B3H_ERR_ARG, "staging overflow");
+  int s = 0;
+  unsigned char *h = next_stage(c, &s);
+  std::memcpy(h, c->sched.entries.data(), ebytes);
+  if (mats) std::memcpy(h + off_m, mats, mbytes);
+  if (times) {
+    std::memcpy(h + off_t, times, tbytes);
+    std::memcpy(h + off_r, rates, rbytes);
+  }
+  unsigned char *d = c->d_stage[s];
+  HIPCHK(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipEventRecord(c->stage_ev[s], c->stream));
+  const Entry *d_entries = reinterpret_cast<const Entry *>(d);
+  const double *d_mats = reinterpret_cast<const double *>(d + off_m);
+  const double *d_times = reinterpret_cast<const double *>(d + off_t);
+  const double *d_rates = reinterpret_cast<const double *>(d + off_r);
+
+  for (uint32_t j = 0; j < S; j++) c->freqs[j] = freqs[j];
+  const int mode = mats ? 0 : 1;
+
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->profiling) {
+    if (c->ev_used == c->ev_start.size()) {
+      hipEvent_t a, b;
+      HIPCHK(hipEventCreate(&a));
+      HIPCHK(hipEventCreate(&b));
+      c->ev_start.push_back(a);
+      c->ev_stop.push_back(b);
+    }
+    e0 = c->ev_start[c->ev_used];
+    e1 = c->ev_stop[c->ev_used];
+    c->ev_used++;
+  }
+
+  if (f32) {
+    HIPCHK(launch_pbuild<float>(d_entries, n, K, S, mode, d_mats, d_times,
+                                d_rates, c->d_model,
+                                static_cast<float *>(c->d_pmats), c->stream));
+    F32Args a{};
+    a.entries = d_entries;
+    a.pmats = static_cast<const float *>(c->d_pmats);
+    a.codes = c->d_codes;
+    a.partials = static_cast<float4 *>(c->d_partials);
+    a.exps = c->d_exps;
+    a.site = c->d_site;
+    a.mixed = c->d_mixed;
+    a.weights = c->d_weights;
+    a.blocksum = c->d_blocksum;
+    a.counter = c->d_counter;
+    a.total = c->d_total;
+    a.pi0 = freqs[0];
+    a.pi1 = freqs[1];
+    a.pi2 = freqs[2];
+    a.pi3 = freqs[3];
+    a.ln_inv_k = std::log(1.0 / (double)K);
+    a.n = n;
+    a.P = c->P;
+    a.K = K;
+    a.T = c->T;
+    a.nslots = std::max<uint32_t>(c->sched.slots_used, 1u);
+    if (e0) HIPCHK(hipEventRecord(e0, c->stream));
+    HIPCHK(launch_prune_f32(a, c->blocks, c->stream));
+    if (e1) HIPCHK(hipEventRecord(e1, c->stream));
+  } else {
+    HIPCHK(launch_pbuild<double>(d_entries, n, K, S, mode, d_mats, d_times,
+                                 d_rates, c->d_model,
+                                 static_cast<double *>(c->d_pmats), c->stream));
+    F64Args a{};
+    a.entries = d_entries;
+    a.pmats = static_cast<const double *>(c->d_pmats);
+    a.codes = c->d_codes;
+    a.partials = static_cast<double4 *>(c->d_partials);
+    a.scales = c->d_scales;
+    a.scale_sums = c->d_scale_sums;
+    a.site = c->d_site;
+    a.pi0 = freqs[0];
+    a.pi1 = freqs[1];
+    a.pi2 = freqs[2];
+    a.pi3 = freqs[3];
+    a.scale_ln = c->scale_ln;
+    a.mult = std::exp((double)c->scale_ln);
+    a.threshold = std::exp(-(double)c->scale_ln);
+    a.n = n;
+    a.P = c->P;
+    a.K = K;
+    a.L = c->L;
+    if (e0) HIPCHK(hipEventRecord(e0, c->stream));
+    HIPCHK(launch_prune_f64(a, c->stream));
+    MixArgs m{};
+    m.site = c->d_site;
+    m.mixed = c->d_mixed;
+    m.weights = c->d_weights;
+    m.blocksum = c->d_mix_blocksum;
+    m.counter = c->d_counter;
+    m.total = c->d_total;
+    m.ln_inv_k = std::log(1.0 / (double)K);
+    m.P = c->P;
+    m.K = K;
+    HIPCHK(launch_mix(m, c->stream));
+    if (e1) HIPCHK(hipEventRecord(e1, c->stream));
+  }
+  c->has_result = true;
+  return B3H_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void b3h_config_default(b3h_config *cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof *cfg);
+  cfg->device = 0;
+  cfg->precision = B3H_F32_SCALED;
+  cfg->states = 4;
+  cfg->scale_ln = 30;
+  cfg->tile_patterns = 0;
+}
+
+const char *b3h_last_error(void) { return g_err.c_str(); }
+
+int b3h_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
+                      uint32_t num_leaves, uint32_t num_patterns,
+                      uint32_t num_categories, b3h_calc **out) {
+  if (!out) return fail(B3H_ERR_ARG, "null output handle pointer");
+  *out = nullptr;
+  b3h_config def;
+  b3h_config_default(&def);
+  if (!cfg) cfg = &def;
+  if (!samples) return fail(B3H_ERR_ARG, "null samples");
+  if (num_leaves < 2) return fail(B3H_ERR_ARG, "need at least 2 leaves");
+  if (num_patterns == 0) return fail(B3H_ERR_ARG, "need at least 1 pattern");
+  if (num_categories == 0 || num_categories > 16)
+    return fail(B3H_ERR_ARG, "num_categories must be 1..16");
+  if (cfg->scale_ln == 0) return fail(B3H_ERR_ARG, "scale_ln must be nonzero");
+  if (cfg->precision != B3H_F32_SCALED && cfg->precision != B3H_F64_PLAIN)
+    return fail(B3H_ERR_UNSUPPORTED, "precision not supported by this entry point");
+  if (cfg->states != 4)
+    return fail(B3H_ERR_UNSUPPORTED, "this entry point supports 4 states");
+  if ((uint64_t)num_leaves * 2 >= (1u << 24))
+    return fail(B3H_ERR_ARG, "too many leaves");
+
+  int ndev = b3h_device_count();
+  if (ndev == 0) return fail(B3H_ERR_NO_DEVICE, "no HIP device visible");
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return fail(B3H_ERR_NO_DEVICE, "device ordinal out of range");
+
+  b3h_calc *c = new b3h_calc();
+  c->device = cfg->device;
+  c->precision = cfg->precision;
+  c->L = num_leaves;
+  c->I = num_leaves - 1;
+  c->P = num_patterns;
+  c->K = num_categories;
+  c->S = 4;
+  c->scale_ln = cfg->scale_ln;
+  c->editbuf.assign(c->I, 0);
+  c->has_result = false;
+
+  auto bail = [&](b3h_status s) {
+    std::string keep = g_err;
+    release(c);
+    g_err = keep;
+    return s;
+  };
+  if (hipSetDevice(c->device) != hipSuccess) {
+    fail(B3H_ERR_HIP, "hipSetDevice failed");
+    return bail(B3H_ERR_HIP);
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    fail(B3H_ERR_HIP, "hipStreamCreate failed");
+    return bail(B3H_ERR_HIP);
+  }
+
+  const uint32_t K = c->K, P = c->P, S = c->S;
+  const size_t slots = (size_t)2 * c->I;
+  const bool f32 = c->precision == B3H_F32_SCALED;
+
+  // tile choice: T sub-tiles of 64 patterns per workgroup, 64*K*T <= 1024
+  uint32_t T = 1;
+  if (cfg->tile_patterns) T = std::max<uint32_t>(1u, cfg->tile_patterns / 64u);
+  else T = std::max<uint32_t>(1u, 4u / K);
+  while (64u * K * T > 1024u && T > 1) T--;
+  c->T = T;
+  c->blocks = (P + 64 * T - 1) / (64 * T);
+
+  // LDS stack capacity for this tile shape
+  uint32_t cap = kMaxSlots;
+  while (cap > 1 && prune_f32_lds_bytes(K, T, cap) > kMaxLds) cap--;
+  c->sched.capacity = cap;
+
+  b3h_status st;
+  std::vector<uint8_t> codes((size_t)c->L * P);
+  for (size_t i = 0; i < codes.size(); i++) codes[i] = remap_code(samples[i], S);
+  if ((st = dev_alloc(c, (void **)&c->d_codes, codes.size()))) return bail(st);
+  const size_t elem = f32 ? 16 : 32;
+  if ((st = dev_alloc(c, &c->d_partials, slots * K * P * elem))) return bail(st);
+  if (f32) {
+    if ((st = dev_alloc(c, (void **)&c->d_exps, slots * K * P * 2))) return bail(st);
+  } else {
+    if ((st = dev_alloc(c, (void **)&c->d_scales, slots * K * P))) return bail(st);
+    if ((st = dev_alloc(c, (void **)&c->d_scale_sums, (size_t)K * P * 4))) return bail(st);
+    if ((st = dev_alloc(c, (void **)&c->d_scale_sums_bak, (size_t)K * P * 4))) return bail(st);
+  }
+  if ((st = dev_alloc(c, (void **)&c->d_site, (size_t)K * P * 8))) return bail(st);
+  if ((st = dev_alloc(c, (void **)&c->d_mixed, (size_t)P * 8))) return bail(st);
+  if ((st = dev_alloc(c, (void **)&c->d_weights, (size_t)P * 4))) return bail(st);
+  if ((st = dev_alloc(c, (void **)&c->d_blocksum, (size_t)c->blocks * 8))) return bail(st);
+  if ((st = dev_alloc(c, (void **)&c->d_mix_blocksum, (size_t)mix_blocks(P) * 8))) return bail(st);
+  if ((st = dev_alloc(c, (void **)&c->d_counter, 64))) return bail(st);
+  if ((st = dev_alloc(c, (void **)&c->d_total, 64))) return bail(st);
+  if ((st = dev_alloc(c, &c->d_pmats, (size_t)c->I * 2 * K * S * S * (f32 ? 4 : 8)))) return bail(st);
+  if ((st = dev_alloc(c, (void **)&c->d_model, sizeof(DevModel)))) return bail(st);
+
+  // staging: entries + f64 matrices (largest mode) + times + rates
+  c->stage_bytes = ((size_t)c->I * sizeof(Entry) + 255) / 256 * 256 +
+                   ((size_t)K * 2 * c->I * S * S * 8 + 255) / 256 * 256 +
+                   ((size_t)2 * c->I * 8 + 255) / 256 * 256 + (size_t)K * 8 + 256;
+  for (int s = 0; s < kStages; s++) {
+    if (hipHostMalloc((void **)&c->h_stage[s], c->stage_bytes, hipHostMallocDefault) != hipSuccess) {
+      fail(B3H_ERR_OOM, "hipHostMalloc staging failed");
+      return bail(B3H_ERR_OOM);
+    }
+    if ((st = dev_alloc(c, (void **)&c->d_stage[s], c->stage_bytes))) return bail(st);
+    if (hipEventCreateWithFlags(&c->stage_ev[s], hipEventDisableTiming) != hipSuccess) {
+      fail(B3H_ERR_HIP, "hipEventCreate failed");
+      return bail(B3H_ERR_HIP);
+    }
+  }
+
+  std::vector<uint32_t> ones(P, 1u);
+  bool ok = hipMemcpy(c->d_codes, codes.data(), codes.size(), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(c->d_weights, ones.data(), (size_t)P * 4, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemset(c->d_counter, 0, 64) == hipSuccess &&
+            hipMemset(c->d_partials, 0, slots * K * P * elem) == hipSuccess;
+  if (ok && f32) ok = hipMemset(c->d_exps, 0, slots * K * P * 2) == hipSuccess;
+  if (ok && !f32)
+    ok = hipMemset(c->d_scales, 0, slots * K * P) == hipSuccess &&
+         hipMemset(c->d_scale_sums, 0, (size_t)K * P * 4) == hipSuccess &&
+         hipMemset(c->d_scale_sums_bak, 0, (size_t)K * P * 4) == hipSuccess;
+  if (ok) ok = hipDeviceSynchronize() == hipSuccess;
+  if (!ok) {
+    fail(B3H_ERR_HIP, "initial device upload failed");
+    return bail(B3H_ERR_HIP);
+  }
+  if (f32 && prepare_prune_f32((int)kMaxLds) != hipSuccess) {
+    fail(B3H_ERR_HIP, "cannot raise the kernel's LDS limit");
+    return bail(B3H_ERR_HIP);
+  }
+  *out = c;
+  return B3H_OK;
+}
+
+b3h_status b3h_destroy(b3h_calc *c) {
+  if (!c) return B3H_OK;
+  (void)hipSetDevice(c->device);
+  release(c);
+  return B3H_OK;
+}
+
+b3h_status b3h_set_weights(b3h_calc *c, const uint32_t *weights) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if (!weights) return fail(B3H_ERR_ARG, "null weights");
+  if ((st = set_device(c))) return st;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->d_weights, weights, (size_t)c->P * 4, hipMemcpyHostToDevice));
+  return B3H_OK;
+}
+
+b3h_status b3h_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
+                       const uint32_t *children, const double *matrices,
+                       const double *frequencies) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if (!matrices || !frequencies) return fail(B3H_ERR_ARG, "null matrices or frequencies");
+  return run_propose(c, n, internals, children, matrices, nullptr, nullptr, frequencies);
+}
+
+b3h_status b3h_set_model(b3h_calc *c, const double *q, const double *freqs,
+                         uint32_t flags) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if (!q || !freqs) return fail(B3H_ERR_ARG, "null model arrays");
+  const uint32_t S = c->S;
+  for (uint32_t i = 0; i < S; i++)
+    if (!(freqs[i] > 0.0)) return fail(B3H_ERR_ARG, "frequencies must be positive");
+  if ((st = set_device(c))) return st;
+  DevModel hm;
+  std::memset(&hm, 0, sizeof hm);
+  hm.n = (int32_t)S;
+  hm.special = (flags & B3H_MODEL_SPECIAL_CASES) ? 1 : 0;
+  for (uint32_t i = 0; i < S; i++) hm.freqs[i] = freqs[i];
+  for (uint32_t i = 0; i < S * S; i++) hm.q[i] = q[i];
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_model, &hm, sizeof hm, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(launch_eigen(c->d_model, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (uint32_t i = 0; i < S; i++) c->freqs[i] = freqs[i];
+  c->model_set = true;
+  return B3H_OK;
+}
+
+b3h_status b3h_propose_times(b3h_calc *c, uint32_t n, const uint32_t *internals,
+                             const uint32_t *children, const double *edge_times,
+                             const double *category_rates) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if (!c->model_set) return fail(B3H_ERR_STATE, "b3h_set_model must be called first");
+  if (!edge_times || !category_rates) return fail(B3H_ERR_ARG, "null edge times or rates");
+  double fr[64];
+  for (uint32_t j = 0; j < c->S; j++) fr[j] = c->freqs[j];
+  return run_propose(c, n, internals, children, nullptr, edge_times, category_rates, fr);
+}
+
+b3h_status b3h_accept(b3h_calc *c) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  for (auto &m : c->editbuf) m &= 1u;  // editbuf.rs:48-53
+  if (c->precision == B3H_F64_PLAIN) {  // cpu4.rs:78-83
+    if ((st = set_device(c))) return st;
+    HIPCHK(hipMemcpyAsync(c->d_scale_sums_bak, c->d_scale_sums, (size_t)c->K * c->P * 4,
+                          hipMemcpyDeviceToDevice, c->stream));
+  }
+  return B3H_OK;
+}
+
+b3h_status b3h_reject(b3h_calc *c) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  for (auto &m : c->editbuf) m = (uint8_t)((m ^ (m >> 1)) & 1u);  // editbuf.rs:55-66
+  if (c->precision == B3H_F64_PLAIN) {  // cpu4.rs:85-89
+    if ((st = set_device(c))) return st;
+    HIPCHK(hipMemcpyAsync(c->d_scale_sums, c->d_scale_sums_bak, (size_t)c->K * c->P * 4,
+                          hipMemcpyDeviceToDevice, c->stream));
+  }
+  return B3H_OK;
+}
+
+b3h_status b3h_likelihoods(b3h_calc *c, uint32_t category, double *out) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if (!out) return fail(B3H_ERR_ARG, "null output");
+  if (category >= c->K) return fail(B3H_ERR_ARG, "category out of range");
+  if (!c->has_result) return fail(B3H_ERR_STATE, "no propose yet");
+  if ((st = set_device(c))) return st;
+  HIPCHK(hipMemcpyAsync(out, c->d_site + (size_t)category * c->P, (size_t)c->P * 8,
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return B3H_OK;
+}
+
+b3h_status b3h_site_mixed(b3h_calc *c, double *out) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if (!out) return fail(B3H_ERR_ARG, "null output");
+  if (!c->has_result) return fail(B3H_ERR_STATE, "no propose yet");
+  if ((st = set_device(c))) return st;
+  HIPCHK(hipMemcpyAsync(out, c->d_mixed, (size_t)c->P * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return B3H_OK;
+}
+
+b3h_status b3h_total(b3h_calc *c, double *out) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if (!out) return fail(B3H_ERR_ARG, "null output");
+  if (!c->has_result) return fail(B3H_ERR_STATE, "no propose yet");
+  if ((st = set_device(c))) return st;
+  HIPCHK(hipMemcpyAsync(out, c->d_total, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return B3H_OK;
+}
+
+uint32_t b3h_num_patterns(const b3h_calc *c) { return c ? c->P : 0; }
+
+uint64_t b3h_device_bytes(const b3h_calc *c) { return c ? c->bytes : 0; }
+
+b3h_status b3h_profile_enable(b3h_calc *c, int enable) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if ((st = set_device(c))) return st;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->profiling = enable != 0;
+  c->ev_used = 0;
+  c->prof_ms = 0.0;
+  c->prof_launches = 0;
+  return B3H_OK;
+}
+
+b3h_status b3h_profile_read(b3h_calc *c, double *kernel_ms, uint64_t *launches) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if ((st = set_device(c))) return st;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < c->ev_used; i++) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev_start[i], c->ev_stop[i]));
+    c->prof_ms += ms;
+    c->prof_launches++;
+  }
+  c->ev_used = 0;
+  if (kernel_ms) *kernel_ms = c->prof_ms;
+  if (launches) *launches = c->prof_launches;
+  return B3H_OK;
+}
+
+}  // extern "C"

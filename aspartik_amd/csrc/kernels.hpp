@@ -1,0 +1,72 @@
+// Host-callable launchers of the b3hip device kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace b3h {
+
+// ---- model.hip ----------------------------------------------------------
+hipError_t launch_eigen(DevModel *m, hipStream_t s);
+template <typename T>
+hipError_t launch_pbuild(const Entry *entries, uint32_t n, uint32_t K,
+                         uint32_t S, int mode, const double *mats,
+                         const double *times, const double *rates,
+                         const DevModel *model, T *out, hipStream_t s);
+
+// ---- prune4.hip ---------------------------------------------------------
+// FP32 partials, 4 states, K categories per workgroup (one wavefront per
+// category and 64-pattern sub-tile), LDS stack for in-pass children,
+// power-of-two normalisation, fused root / category mix / weighted sum.
+struct F32Args {
+  const Entry *entries;
+  const float *pmats;     // [n][2][K][16]
+  const uint8_t *codes;   // [L][P] state codes 0..4
+  float4 *partials;       // [slots][K][P]
+  int16_t *exps;          // [slots][K][P] cumulative binary exponents
+  double *site;           // [K][P]
+  double *mixed;          // [P]
+  const uint32_t *weights;
+  double *blocksum;       // [gridDim.x]
+  uint32_t *counter;      // last-block ticket (self-resetting)
+  double *total;
+  double pi0, pi1, pi2, pi3;
+  double ln_inv_k;
+  uint32_t n, P, K, T, nslots;
+};
+hipError_t launch_prune_f32(const F32Args &a, uint32_t blocks, hipStream_t s);
+size_t prune_f32_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots);
+hipError_t prepare_prune_f32(int max_lds_bytes);
+
+// ---- prune64.hip --------------------------------------------------------
+// FP64 plain baseline: one lane per (pattern, category), b3's e^scale_ln
+// flag scaling, children read from HBM.
+struct F64Args {
+  const Entry *entries;
+  const double *pmats;    // [n][2][K][16]
+  const uint8_t *codes;   // [L][P]
+  double4 *partials;      // [slots][K][P]
+  uint8_t *scales;        // [slots][K][P]
+  uint32_t *scale_sums;   // [K][P]
+  double *site;           // [K][P]
+  double pi0, pi1, pi2, pi3;
+  double threshold, mult;
+  uint32_t scale_ln;
+  uint32_t n, P, K, L;
+};
+hipError_t launch_prune_f64(const F64Args &a, hipStream_t s);
+
+// Category mix + weighted sum with a deterministic last-block reduction.
+struct MixArgs {
+  const double *site;  // [K][P]
+  double *mixed;       // [P]
+  const uint32_t *weights;
+  double *blocksum;
+  uint32_t *counter;
+  double *total;
+  double ln_inv_k;
+  uint32_t P, K;
+};
+hipError_t launch_mix(const MixArgs &a, hipStream_t s);
+uint32_t mix_blocks(uint32_t P);
+
+}  // namespace b3h

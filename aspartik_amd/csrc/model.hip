@@ -1,0 +1,166 @@
+// Device-side substitution model: FP64 eigensystem of a reversible Q and
+// per-edge transition matrices P(t) = V diag(exp(lambda t)) V^-1.
+//
+// Replaces the host-side eigendecomposition (crates/b3/src/beast_eigen.rs:
+// 66-101, called from substitution.rs:544-554) and the per-edge host build
+// (transitions.rs:26-48, substitution.rs:403-420,571-588).  Q is reversible
+// for every model b3 exports, so S = D^1/2 Q D^-1/2 (D = diag(pi)) is
+// symmetric and is diagonalised by cyclic Jacobi rotations in one wavefront.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace b3h {
+
+// One wavefront, n <= 64.  A and U live in LDS (2 x 32 KiB).  Lane k owns
+// row/column k of every rotation, so each rotation costs a handful of LDS
+// operations; rotations of a sweep are applied in the cyclic (p, q) order.
+__global__ __launch_bounds__(64) void eigen_kernel(DevModel *m) {
+  __shared__ double A[64 * 64];
+  __shared__ double U[64 * 64];
+  const int n = m->n;
+  const int k = threadIdx.x;
+  if (k < n) {
+    const double sk = sqrt(m->freqs[k]);
+    for (int j = 0; j < n; j++) {
+      const double sj = sqrt(m->freqs[j]);
+      const double skj = m->q[k * n + j] * sk / sj;
+      const double sjk = m->q[j * n + k] * sj / sk;
+      A[k * 64 + j] = 0.5 * (skj + sjk);
+      U[k * 64 + j] = (k == j) ? 1.0 : 0.0;
+    }
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < 64; sweep++) {
+    // convergence test: off-diagonal mass relative to total
+    double off = 0.0, tot = 0.0;
+    if (k < n)
+      for (int j = 0; j < n; j++) {
+        const double v = A[k * 64 + j] * A[k * 64 + j];
+        tot += v;
+        if (j != k) off += v;
+      }
+    for (int s = 32; s > 0; s >>= 1) {
+      off += __shfl_xor(off, s);
+      tot += __shfl_xor(tot, s);
+    }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    int rotated = 0;
+    for (int p = 0; p < n - 1; p++) {
+      for (int q = p + 1; q < n; q++) {
+        const double apq = A[p * 64 + q];
+        const double app = A[p * 64 + p], aqq = A[q * 64 + q];
+        // uniform: every lane read the same values.  Rotations too small to
+        // change the diagonal at double precision are skipped.
+        if (fabs(apq) <= 1e-18 * (fabs(app) + fabs(aqq)) || apq == 0.0) continue;
+        rotated = 1;
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) /
+                         (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        __syncthreads();
+        if (k < n) {  // columns p, q
+          const double akp = A[k * 64 + p], akq = A[k * 64 + q];
+          A[k * 64 + p] = c * akp - s * akq;
+          A[k * 64 + q] = s * akp + c * akq;
+        }
+        __syncthreads();
+        if (k < n) {  // rows p, q
+          const double apk = A[p * 64 + k], aqk = A[q * 64 + k];
+          A[p * 64 + k] = c * apk - s * aqk;
+          A[q * 64 + k] = s * apk + c * aqk;
+          const double ukp = U[k * 64 + p], ukq = U[k * 64 + q];
+          U[k * 64 + p] = c * ukp - s * ukq;
+          U[k * 64 + q] = s * ukp + c * ukq;
+        }
+        __syncthreads();
+      }
+    }
+    if (!rotated) break;
+  }
+  __syncthreads();
+  if (k < n) {
+    const double sk = sqrt(m->freqs[k]);
+    for (int j = 0; j < n; j++) {
+      m->evec[k * n + j] = U[k * 64 + j] / sk;   // V = D^-1/2 U
+      m->ievc[j * n + k] = U[k * 64 + j] * sk;   // V^-1 = U^T D^1/2
+    }
+    m->eval[k] = A[k * 64 + k];
+  }
+}
+
+// P(t) element (r, col) for an S-state model: the special cases of
+// substitution.rs:245-268,408-414 (when enabled), else the eigen product.
+__device__ inline double p_element(const DevModel *m, int S, double t, int r,
+                                   int col) {
+  if (m->special) {
+    if (t > 20.0) return m->freqs[col];
+    if (t < 1e-10) return (r == col) ? 1.0 : m->q[r * S + col] * t;
+  }
+  double acc = 0.0;
+  for (int k = 0; k < S; k++)
+    acc += m->evec[r * S + k] * exp(m->eval[k] * t) * m->ievc[k * S + col];
+  return acc;
+}
+
+// Build the transition matrices of one propose.  Output layout
+// [entry][side][category][S*S] in the kernels' entry order; `meta >> 8` maps
+// an entry to the caller's list position.  Mode 0 casts host matrices given
+// as [K][2n][S*S] f64; mode 1 builds them from edge_times[2n] x rates[K].
+template <typename T>
+__global__ void pbuild_kernel(const Entry *entries, uint32_t n, uint32_t K,
+                              uint32_t S, int mode, const double *mats,
+                              const double *times, const double *rates,
+                              const DevModel *model, T *out) {
+  const uint32_t SS = S * S;
+  const uint64_t total = (uint64_t)n * 2 * K * SS;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t el = (uint32_t)(g % SS);
+    const uint64_t mat = g / SS;  // (entry * 2 + side) * K + c
+    const uint32_t c = (uint32_t)(mat % K);
+    const uint64_t es = mat / K;
+    const uint32_t side = (uint32_t)(es & 1);
+    const uint32_t e = (uint32_t)(es >> 1);
+    const uint32_t orig = entries[e].meta >> 8;
+    double v;
+    if (mode == 0) {
+      v = mats[((uint64_t)c * 2 * n + 2ull * orig + side) * SS + el];
+    } else {
+      const double t = times[2 * orig + side] * rates[c];
+      v = p_element(model, (int)S, t, (int)(el / S), (int)(el % S));
+    }
+    out[g] = (T)v;
+  }
+}
+
+hipError_t launch_eigen(DevModel *m, hipStream_t s) {
+  hipLaunchKernelGGL(eigen_kernel, dim3(1), dim3(64), 0, s, m);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_pbuild(const Entry *entries, uint32_t n, uint32_t K,
+                         uint32_t S, int mode, const double *mats,
+                         const double *times, const double *rates,
+                         const DevModel *model, T *out, hipStream_t s) {
+  const uint64_t total = (uint64_t)n * 2 * K * S * S;
+  uint64_t blocks = (total + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(pbuild_kernel<T>, dim3((uint32_t)blocks), dim3(256), 0, s,
+                     entries, n, K, S, mode, mats, times, rates, model, out);
+  return hipGetLastError();
+}
+
+template hipError_t launch_pbuild<float>(const Entry *, uint32_t, uint32_t,
+                                         uint32_t, int, const double *,
+                                         const double *, const double *,
+                                         const DevModel *, float *,
+                                         hipStream_t);
+template hipError_t launch_pbuild<double>(const Entry *, uint32_t, uint32_t,
+                                          uint32_t, int, const double *,
+                                          const double *, const double *,
+                                          const DevModel *, double *,
+                                          hipStream_t);
+
+}  // namespace b3h

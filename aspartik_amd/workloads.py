@@ -1,0 +1,107 @@
+"""Seeded synthetic workloads of the measured configurations.
+
+Each builder returns a dict with the tree arrays (b3 numbering), the
+substitution model object, IUPAC codes [L, P], pattern weights, and the
+discrete-gamma settings.  Unspecified choices are stated here: Yule birth
+rate 1 with branch lengths rescaled to mean 0.1; seed 4 by default.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import sim
+from .b3.calculator import gamma_rates
+from .b3.substitution import GTR, HKY
+
+
+def gtr_gamma4(num_leaves=128, num_patterns=100_000, alpha=0.5, seed=4, weights=(1, 5)):
+    """GTR+Gamma4: GTR rates ~ Dirichlet(1), freqs ~ Dirichlet(10), alpha 0.5,
+    ``num_patterns`` unique patterns simulated on a Yule tree, integer
+    weights uniform on [1, 5]."""
+    rng = np.random.default_rng(seed)
+    parents, children, heights = sim.yule_tree(num_leaves, rng, 1.0, 0.1)
+    freqs = rng.dirichlet([10.0] * 4)
+    rates6 = rng.dirichlet([1.0] * 6)
+    model = GTR(freqs, rates6)
+    K = 4
+    cat = gamma_rates(alpha, K)
+    states = sim.simulate_unique_patterns(parents, children, heights,
+                                          sim.expm_transition(model.q_matrix()), freqs, cat,
+                                          num_patterns, rng)
+    codes = sim.dna_codes(states)
+    lo, hi = weights
+    w = rng.integers(lo, hi + 1, size=num_patterns).astype(np.uint32)
+    return dict(name=f"gtr_gamma4_{num_leaves}taxa_{num_patterns}pat", parents=parents,
+                children=children, heights=heights, model=model, codes=codes, weights=w,
+                K=K, alpha=alpha, rates=cat, seed=seed)
+
+
+def hky_gamma4(num_leaves, num_patterns, seed, alpha=0.5):
+    """HKY+Gamma4 instance: kappa ~ LogNormal(1, 0.5), freqs 0.25, weights 1."""
+    rng = np.random.default_rng(seed)
+    parents, children, heights = sim.yule_tree(num_leaves, rng, 1.0, 0.1)
+    kappa = float(rng.lognormal(1.0, 0.5))
+    freqs = np.full(4, 0.25)
+    model = HKY(freqs, kappa)
+    K = 4
+    cat = gamma_rates(alpha, K)
+    states = sim.simulate_unique_patterns(parents, children, heights,
+                                          sim.expm_transition(model.q_matrix()), freqs, cat,
+                                          num_patterns, rng)
+    return dict(name=f"hky_gamma4_{num_leaves}taxa_{num_patterns}pat", parents=parents,
+                children=children, heights=heights, model=model, codes=sim.dna_codes(states),
+                weights=np.ones(num_patterns, np.uint32), K=K, alpha=alpha, rates=cat,
+                seed=seed)
+
+
+def full_lists(tree):
+    """Update list of a full evaluation (every edge marked)."""
+    tree.mark_all_edges_updated()
+    internals, children = tree.partials_lists()
+    tree.accept()
+    return internals, children
+
+
+def path_lists(tree, edge):
+    """Update list after changing one edge: the edge's ancestors to the root."""
+    tree.mark_edge_updated(edge)
+    internals, children = tree.partials_lists()
+    tree.accept()
+    return internals, children
+
+
+def edge_times(tree, children):
+    flat = np.asarray(children).reshape(-1).astype(np.int64)
+    return tree.heights[tree.parents[flat]] - tree.heights[flat]
+
+
+def algorithmic_bytes(internals, children, num_leaves, root, K, P, states=4, elem=4):
+    """Algorithmic HBM bytes of one evaluation (each child partial read, each
+    non-root result written; tip codes counted once across K; plus weights
+    and the 8-byte result):
+      B = sum_n [sum_child (leaf ? P : K P S b) + (n != root ? K P S b : 0)]
+          + 4 P + 8
+    """
+    part = K * P * states * elem
+    B = 0
+    for i, n in enumerate(np.asarray(internals)):
+        for c in np.asarray(children)[i]:
+            B += P if c < num_leaves else part
+        if n != root:
+            B += part
+    return B + 4 * P + 8
+
+
+def algorithmic_flops(internals, children, num_leaves, K, P, states=4):
+    """Per (node, pattern, category): sum over internal children of 2 S^2,
+    plus S for the product; at the root 2S + 1 more."""
+    f = 0
+    S = states
+    for i in range(len(internals)):
+        per = S
+        for c in np.asarray(children)[i]:
+            if c >= num_leaves:
+                per += 2 * S * S
+        f += per
+    f += 2 * S + 1
+    return f * K * P

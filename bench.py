@@ -1,0 +1,242 @@
+#!/usr/bin/env python3
+"""Benchmark of the FP32-scaled pruning likelihood on one MI355X.
+
+Workload: GTR+Gamma4, 128 taxa (seeded Yule tree, mean branch 0.1),
+100,000 unique simulated patterns with weights U{1..5}.  One step is one
+full likelihood evaluation (every edge dirty): update list + edge times
+uploaded, P(t) built on the GPU, post-order pruning over all 127 internal
+nodes for 4 categories, FP64 root / category mix / weighted sum, the total
+copied back.  Also reported: the same evaluation on the FP64 plain GPU
+kernel (the baseline of the speed-up target), partial evaluations (one edge
+to the root), the error against the FP64 CPU oracle, and the oracle's own
+throughput on the host cores (cpu_baseline).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "logL evals/s vs plain FP64 at <=1e-6 rel err; HBM GB/s; ESS/s vs plain MCMC"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic(workload):
+    """HBM bytes per launch of the pruning kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    kernel calibration), if one exists for this workload."""
+    path = os.path.join(ROOT, "profiles", f"prune_f32_{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def time_evals(calc, lists, times, rates, steps, warmup, sync_each=True):
+    internals, children = lists
+    for _ in range(warmup):
+        calc.propose_times(internals, children, times, rates)
+        calc.total()
+    calc.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        calc.propose_times(internals, children, times, rates)
+        if sync_each:
+            calc.total()
+    total = calc.total()
+    dt = time.perf_counter() - t0
+    kms, launches = calc.profile_read()
+    calc.profile_enable(False)
+    return dt, total, kms / max(1, launches)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--taxa", type=int, default=128)
+    ap.add_argument("--patterns", type=int, default=100_000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-f32", action="store_true", help="profiling runs: FP32 kernel only")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the likelihood shards by pattern with one 8-byte all-reduce per "
+              "evaluation; that exchange is not built).", file=sys.stderr)
+        sys.exit(2)
+
+    from aspartik_amd import workloads
+    from aspartik_amd.b3 import Calculator, DatedTree
+
+    log("[bench] building workload")
+    wl = workloads.gtr_gamma4(args.taxa, args.patterns)
+    L, P, K = args.taxa, args.patterns, wl["K"]
+    tree = DatedTree(L, wl["parents"], wl["children"], wl["heights"])
+    full = workloads.full_lists(tree)
+    times_full = workloads.edge_times(tree, full[1])
+    rates = wl["rates"]
+    model = wl["model"]
+    workload_name = f"gtr_gamma4_{L}taxa_{P}pat"
+
+    log("[bench] FP32 calculator")
+    calc = Calculator.HIP().make(wl["codes"], P, K)
+    calc.set_weights(wl["weights"])
+    calc.set_model(model.q_matrix(), model.frequencies(), True)
+    dt, total32, kernel_ms = time_evals(calc, full, times_full, rates, args.steps, args.warmup)
+    evals_per_s = args.steps / dt
+    log(f"[bench] fp32 full: {evals_per_s:.1f} evals/s, kernel {kernel_ms * 1e3:.1f} us")
+
+    # pipelined (no host round trip per evaluation)
+    dt_pipe, _, _ = time_evals(calc, full, times_full, rates, args.steps, 2, sync_each=False)
+
+    # partial evaluations: one random non-root edge and its path to the root
+    rng = np.random.default_rng(99)
+    root = tree.root
+    edges = [e for e in rng.choice(2 * L - 2, size=64) if e != root]
+    plists = [workloads.path_lists(tree, int(e)) for e in edges]
+    ptimes = [workloads.edge_times(tree, pl[1]) for pl in plists]
+    for i in range(10):
+        calc.propose_times(*plists[i % len(plists)], ptimes[i % len(plists)], rates)
+        calc.total()
+        calc.accept()
+    calc.profile_enable(True)
+    nparts = max(50, args.steps)
+    t0 = time.perf_counter()
+    for i in range(nparts):
+        j = i % len(plists)
+        calc.propose_times(*plists[j], ptimes[j], rates)
+        calc.total()
+        calc.accept()
+    dt_part = time.perf_counter() - t0
+    pk_ms, pk_n = calc.profile_read()
+    calc.profile_enable(False)
+    mean_path = float(np.mean([pl[0].size for pl in plists]))
+    log(f"[bench] fp32 partial: {nparts / dt_part:.1f} evals/s (mean path {mean_path:.1f} nodes)")
+    # restore the full state for the error check
+    calc.propose_times(*full, times_full, rates)
+    total32 = calc.total()
+    site32 = calc.site_mixed()
+    dev_bytes = calc.device_bytes()
+    calc.close()
+
+    f64 = None
+    if not args.only_f32:
+        log("[bench] FP64 plain calculator")
+        c64 = Calculator.HIPPlainF64().make(wl["codes"], P, K)
+        c64.set_weights(wl["weights"])
+        c64.set_model(model.q_matrix(), model.frequencies(), True)
+        steps64 = max(20, args.steps // 4)
+        dt64, total64, k64 = time_evals(c64, full, times_full, rates, steps64, 5)
+        f64 = dict(evals_per_s=steps64 / dt64, kernel_us=k64 * 1e3, total=total64)
+        c64.close()
+        log(f"[bench] fp64 plain: {f64['evals_per_s']:.1f} evals/s")
+
+    # algorithmic bytes / flops of one full evaluation
+    B = workloads.algorithmic_bytes(full[0], full[1], L, root, K, P)
+    F = workloads.algorithmic_flops(full[0], full[1], L, K, P)
+    achieved = B / (kernel_ms * 1e-3) / 1e9
+    traffic = read_traffic(workload_name)
+
+    cpu = None
+    err = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import oracle as O
+        log("[bench] FP64 CPU oracle (cpu_baseline)")
+        otree = O.Tree(L, wl["parents"], wl["children"], wl["heights"])
+        om = O.Model.gtr(model.rates, model.freqs)
+        ncores = len(os.sched_getaffinity(0))
+        tpc = min(ncores, (P + 999) // 1000)  # cpu4.rs:127-132
+        t0 = time.perf_counter()
+        olik = O.GammaLikelihood(otree, om, wl["codes"], P, wl["weights"], num_categories=K,
+                                 alpha=wl["alpha"], threads_per_category=tpc,
+                                 parallel_categories=True)
+        t_first = time.perf_counter() - t0
+        ref_total = olik.likelihood()
+        ref_sites = None
+        # per-pattern mixed values for the error statement
+        site = np.array([olik.site(c) for c in range(K)])
+        mx = site.max(0)
+        ref_sites = mx + np.log(np.exp(site - mx).sum(0)) + np.log(1.0 / K)
+        nev = 0
+        t0 = time.perf_counter()
+        while True:
+            otree.mark_all_edges_updated()
+            olik.likelihood()
+            olik.accept()
+            otree.accept()
+            nev += 1
+            if time.perf_counter() - t0 > args.cpu_seconds or nev >= 200:
+                break
+        dtc = time.perf_counter() - t0
+        cpu = {"value": nev / dtc, "unit": "logL evals/s", "cores": tpc * K,
+               "kind": "port",
+               "sample": f"{nev} full GTR+Gamma4 evaluations of the same {L}-taxon "
+                         f"{P}-pattern workload ({dtc:.1f} s; {K} categories x {tpc} "
+                         f"threads as cpu4.rs/gamma.rs; host has {ncores} cores; "
+                         f"construction {t_first:.1f} s)"}
+        rel_total = abs(total32 - ref_total) / abs(ref_total)
+        per_site = np.abs(site32 - ref_sites) / np.maximum(1.0, np.abs(ref_sites))
+        err = dict(total_rel=rel_total, max_site=float(per_site.max()), ref_total=ref_total,
+                   gpu_total=total32)
+        if f64 is not None:
+            err["fp64_gpu_total_rel"] = abs(f64["total"] - ref_total) / abs(ref_total)
+        log(f"[bench] oracle {cpu['value']:.3f} evals/s; rel err total {rel_total:.2e}")
+
+    line = {
+        "metric": METRIC,
+        "value": evals_per_s,
+        "unit": "logL evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Yule tree, sequences simulated along it)",
+        "config": {"workload": workload_name, "taxa": L, "patterns": P, "categories": K,
+                   "model": "GTR+Gamma4 alpha 0.5", "evaluation": "full (all edges dirty)",
+                   "seed": wl["seed"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+        "kernel_us": kernel_ms * 1e3,
+        "algorithmic_bytes_per_eval": B,
+        "algorithmic_flops_per_eval": F,
+        "pipelined_evals_per_s": args.steps / dt_pipe,
+        "partial_evals_per_s": nparts / dt_part,
+        "partial_kernel_us": pk_ms / max(1, pk_n) * 1e3,
+        "partial_mean_path_nodes": mean_path,
+        "partial_updates_per_s": evals_per_s * len(full[0]) * P * K,
+        "fp64_plain": f64,
+        "speedup_vs_fp64_plain": (evals_per_s / f64["evals_per_s"]) if f64 else None,
+        "error_vs_fp64_oracle": err,
+        "device_bytes": dev_bytes,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

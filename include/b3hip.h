@@ -1,0 +1,184 @@
+/*
+ * b3hip.h -- C ABI of the MI355X (gfx950) Felsenstein-pruning calculator.
+ *
+ * This is the drop-in boundary for b3's `Calculator<f64>` trait
+ * (crates/b3/src/calculator/mod.rs:54-76).  One handle plays the role of the
+ * K calculators that GammaLikelihood keeps, one per rate category
+ * (crates/b3/src/likelihood/gamma.rs:57-66); with K = 1 it is exactly the one
+ * calculator DnaLikelihood keeps (crates/b3/src/likelihood/dna.rs:41-42).
+ *
+ * Conventions
+ *  - Plain C types only.  Every call returns a b3h_status; no call aborts or
+ *    throws.  On failure b3h_last_error() gives a message (thread-local).
+ *  - Node numbering is b3's: leaves 0..L-1, internal nodes L..2L-2; the edge
+ *    above node n has index n (crates/b3/src/parameters/tree.rs:728-743).
+ *  - A handle owns its device buffers and one HIP stream.  Calls on one handle
+ *    must be serialised by the caller (b3 holds each calculator behind a
+ *    mutex, dna.rs:18); distinct handles may be used from distinct threads
+ *    concurrently (gamma.rs:143-152).  No global mutable state.
+ *  - Protocol (calculator/mod.rs:44-53): propose -> (likelihoods | total) ->
+ *    exactly one of accept / reject.
+ */
+#ifndef B3HIP_H
+#define B3HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum b3h_status {
+  B3H_OK = 0,
+  B3H_ERR_ARG = 1,         /* invalid argument (null, size, node id, ...)   */
+  B3H_ERR_NO_DEVICE = 2,   /* no HIP device / device ordinal out of range   */
+  B3H_ERR_OOM = 3,         /* device or pinned-host allocation failed       */
+  B3H_ERR_HIP = 4,         /* a HIP runtime call failed                     */
+  B3H_ERR_STATE = 5,       /* call out of protocol order                    */
+  B3H_ERR_UNSUPPORTED = 6  /* combination not implemented                   */
+} b3h_status;
+
+/* Arithmetic of the pruning kernels. */
+typedef enum b3h_precision {
+  /* FP32 partials with exact per-node power-of-two rescaling, FP64
+   * exponents / root / mixing / weighted sum.  The product path. */
+  B3H_F32_SCALED = 0,
+  /* FP64 partials, one lane per (pattern, category), children read from
+   * HBM, e^scale_ln threshold scaling with double-buffered flags: b3's own
+   * GPU algorithm (cuda_partials/kernels.cu:36-140) at FP64, kept as the
+   * plain-FP64 baseline. */
+  B3H_F64_PLAIN = 1,
+  /* Codon / S-state models: partials FP16 in HBM, P x partials on MFMA
+   * (fp16 in, fp32 accumulate), power-of-two scaling. */
+  B3H_F16_MFMA = 2
+} b3h_precision;
+
+typedef struct b3h_config {
+  int32_t device;          /* HIP device ordinal                              */
+  uint32_t precision;      /* b3h_precision                                   */
+  uint32_t states;         /* 4 (DNA) or 5..64 (e.g. 61 codons)               */
+  uint32_t scale_ln;       /* calculator/mod.rs:108,149-155 (default 30); the
+                              F64_PLAIN path scales by e^scale_ln.  The FP32
+                              and FP16 paths scale by exact powers of two and
+                              only check it is nonzero.                       */
+  uint32_t tile_patterns;  /* patterns per workgroup tile; 0 = tuned default  */
+  uint32_t reserved[3];
+} b3h_config;
+
+typedef struct b3h_calc b3h_calc;
+
+/* Fill *cfg with defaults: device 0, B3H_F32_SCALED, 4 states, scale_ln 30. */
+void b3h_config_default(b3h_config *cfg);
+
+/*
+ * Create a calculator.  Replaces CalculatorConfig::make4 (calculator/mod.rs:
+ * 159-200) and Cpu4Calculator::new (calculator/cpu4.rs:108-165).
+ *   samples: [num_leaves x num_patterns] bytes, row-major by leaf, as
+ *            deduplicate() produces them (likelihood/mod.rs:124-129).  For
+ *            4 states these are IUPAC bit codes: A=1 C=2 G=4 T=8; any other
+ *            value is a fully unknown state (cpu4.rs:117-125).  For S > 4
+ *            they are state indices 0..S-1; values >= S are unknown.
+ *   num_categories: K rate categories evaluated by this one handle.
+ */
+b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
+                      uint32_t num_leaves, uint32_t num_patterns,
+                      uint32_t num_categories, b3h_calc **out);
+
+b3h_status b3h_destroy(b3h_calc *calc);
+
+/* Pattern weights used by b3h_total (likelihood/mod.rs:134-139 and
+ * gamma.rs:154-166).  Default: all ones. */
+b3h_status b3h_set_weights(b3h_calc *calc, const uint32_t *weights);
+
+/*
+ * Calculator::propose (calculator/mod.rs:56-60; cpu4.rs:39-76).
+ *   internals[n], children[2n]: DatedTree::partials_lists() (tree.rs:456-461):
+ *       dirty internal nodes, children before parents, root last; each
+ *       child pair sorted ascending.
+ *   matrices: [K][2n][S][S] f64 row-major, category-major; within a
+ *       category, Transitions::write_matrices(children) order
+ *       (transitions.rs:62-72).
+ *   frequencies: [S] f64 (Transitions::frequencies, transitions.rs:50-52).
+ * Enqueues the work and returns; results are read by b3h_likelihoods or
+ * b3h_total.
+ */
+b3h_status b3h_propose(b3h_calc *calc, uint32_t n, const uint32_t *internals,
+                       const uint32_t *children, const double *matrices,
+                       const double *frequencies);
+
+/*
+ * Substitution model on the device: the eigensystem of Q is computed on the
+ * GPU in FP64 (replaces the host-side SubstitutionModel::update and
+ * DefaultEigenSystem::decompose_matrix, substitution.rs:336-388,502-555 and
+ * beast_eigen.rs:66-101).
+ *   q: [S][S] f64, a reversible rate matrix normalised to mean rate 1.
+ *   frequencies: [S] f64 stationary distribution.
+ *   flags: B3H_MODEL_SPECIAL_CASES applies HKY/GTR's t > 20 -> rows = pi and
+ *          t < 1e-10 -> I + Qt with unit diagonal (substitution.rs:245-268,
+ *          408-414, 576-582).
+ */
+#define B3H_MODEL_SPECIAL_CASES 1u
+b3h_status b3h_set_model(b3h_calc *calc, const double *q,
+                         const double *frequencies, uint32_t flags);
+
+/*
+ * propose() with P(t) built on the GPU from the device eigensystem
+ * (replaces Transitions::update + write_matrices, transitions.rs:26-72):
+ *   edge_times[2n]: time lengths of the child edges in children order
+ *       (tree.rs:732-736, h_parent - h_child).
+ *   category_rates[K]: clock_rate x r_c per category (gamma.rs:126-134).
+ *   The branch length of child m in category c is edge_times[m] *
+ *   category_rates[c], exactly as transitions.rs:34-35 forms it.
+ */
+b3h_status b3h_propose_times(b3h_calc *calc, uint32_t n,
+                             const uint32_t *internals,
+                             const uint32_t *children,
+                             const double *edge_times,
+                             const double *category_rates);
+
+/* Calculator::accept / reject (calculator/mod.rs:62-69; cpu4.rs:78-89). */
+b3h_status b3h_accept(b3h_calc *calc);
+b3h_status b3h_reject(b3h_calc *calc);
+
+/* Calculator::likelihoods (calculator/mod.rs:71-72): per-pattern ln L of
+ * category k, net of scaling.  Blocks until the last propose finishes. */
+b3h_status b3h_likelihoods(b3h_calc *calc, uint32_t category, double *out);
+
+/* Per-pattern mixed value LSE_c(lnL_c) + ln(1/K) (gamma.rs:159-163); for
+ * K = 1 this equals b3h_likelihoods(calc, 0, ...). */
+b3h_status b3h_site_mixed(b3h_calc *calc, double *out);
+
+/* The likelihood layer's total: sum_i w_i * mixed_i, reduced in FP64 on the
+ * device (dna.rs:96-97, gamma.rs:156-166).  Blocks. */
+b3h_status b3h_total(b3h_calc *calc, double *out);
+
+/* Calculator::num_patterns (calculator/mod.rs:74-75). */
+uint32_t b3h_num_patterns(const b3h_calc *calc);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *b3h_last_error(void);
+
+/* Number of visible HIP devices (0 when none; never fails). */
+int b3h_device_count(void);
+
+/* Kernel timing for measurement: when enabled, every pruning-kernel launch
+ * is bracketed by HIP events on the handle's stream.  read() synchronises
+ * and returns the summed kernel time and the launch count since enable. */
+b3h_status b3h_profile_enable(b3h_calc *calc, int enable);
+b3h_status b3h_profile_read(b3h_calc *calc, double *kernel_ms,
+                            uint64_t *launches);
+
+/* Discrete-gamma category rates (likelihood/gamma.rs:205-225): the
+ * Gamma(alpha, 1) quantiles at (2i+1)/(2K), i = 0..K-1, divided by their
+ * mean.  Host-side FP64; K = 1 gives rate 1. */
+b3h_status b3h_gamma_rates(double alpha, uint32_t num_categories,
+                           double *rates);
+
+/* Bytes of device memory held by the handle. */
+uint64_t b3h_device_bytes(const b3h_calc *calc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
