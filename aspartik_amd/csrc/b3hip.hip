@@ -45,6 +45,7 @@ struct b3h_calc {
   uint32_t L = 0, I = 0, P = 0, K = 0, S = 4;
   uint32_t scale_ln = 30;
   uint32_t T = 1;  // 64-pattern sub-tiles per workgroup
+  uint32_t cstride = 0;  // device code row stride (P rounded up to 64)
   uint32_t blocks = 0;
 
   // device buffers
@@ -248,7 +249,8 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     F32Args a{};
     a.entries = d_entries;
     a.pmats = static_cast<const float *>(c->d_pmats);
-    a.codes = c->d_codes;
+    a.codes32 = reinterpret_cast<const uint32_t *>(c->d_codes);
+    a.cstride = c->cstride;
     a.partials = static_cast<float4 *>(c->d_partials);
     a.exps = c->d_exps;
     a.site = c->d_site;
@@ -278,6 +280,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     a.entries = d_entries;
     a.pmats = static_cast<const double *>(c->d_pmats);
     a.codes = c->d_codes;
+    a.cstride = c->cstride;
     a.partials = static_cast<double4 *>(c->d_partials);
     a.scales = c->d_scales;
     a.scale_sums = c->d_scale_sums;
@@ -402,14 +405,21 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   c->T = T;
   c->blocks = (P + 64 * T - 1) / (64 * T);
 
-  // LDS stack capacity for this tile shape
+  // LDS stack capacity for this tile shape and the longest update list
   uint32_t cap = kMaxSlots;
-  while (cap > 1 && prune_f32_lds_bytes(K, T, cap) > kMaxLds) cap--;
+  while (cap > 1 && prune_f32_lds_bytes(K, T, cap, c->I) > kMaxLds) cap--;
+  if (prune_f32_lds_bytes(K, T, cap, c->I) > kMaxLds) {
+    fail(B3H_ERR_UNSUPPORTED, "tree too large for the LDS-staged update list");
+    return bail(B3H_ERR_UNSUPPORTED);
+  }
   c->sched.capacity = cap;
 
   b3h_status st;
-  std::vector<uint8_t> codes((size_t)c->L * P);
-  for (size_t i = 0; i < codes.size(); i++) codes[i] = remap_code(samples[i], S);
+  c->cstride = (P + 64 * T - 1) / (64 * T) * (64 * T);  // whole tiles, no over-read
+  std::vector<uint8_t> codes((size_t)c->L * c->cstride, (uint8_t)S);
+  for (size_t l = 0; l < c->L; l++)
+    for (size_t i = 0; i < P; i++)
+      codes[l * c->cstride + i] = remap_code(samples[l * P + i], S);
   if ((st = dev_alloc(c, (void **)&c->d_codes, codes.size()))) return bail(st);
   const size_t elem = f32 ? 16 : 32;
   if ((st = dev_alloc(c, &c->d_partials, slots * K * P * elem))) return bail(st);

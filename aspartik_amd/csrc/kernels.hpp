@@ -20,7 +20,7 @@ hipError_t launch_pbuild(const Entry *entries, uint32_t n, uint32_t K,
 struct F32Args {
   const Entry *entries;
   const float *pmats;     // [n][2][K][16]
-  const uint8_t *codes;   // [L][P] state codes 0..4
+  const uint32_t *codes32;  // [L][cstride] state codes 0..4, 4 per dword
   float4 *partials;       // [slots][K][P]
   int16_t *exps;          // [slots][K][P] cumulative binary exponents
   double *site;           // [K][P]
@@ -32,9 +32,10 @@ struct F32Args {
   double pi0, pi1, pi2, pi3;
   double ln_inv_k;
   uint32_t n, P, K, T, nslots;
+  uint32_t cstride;       // code row stride (a multiple of 64 T)
 };
 hipError_t launch_prune_f32(const F32Args &a, uint32_t blocks, hipStream_t s);
-size_t prune_f32_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots);
+size_t prune_f32_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n);
 hipError_t prepare_prune_f32(int max_lds_bytes);
 
 // ---- prune64.hip --------------------------------------------------------
@@ -43,7 +44,7 @@ hipError_t prepare_prune_f32(int max_lds_bytes);
 struct F64Args {
   const Entry *entries;
   const double *pmats;    // [n][2][K][16]
-  const uint8_t *codes;   // [L][P]
+  const uint8_t *codes;   // [L][cstride]
   double4 *partials;      // [slots][K][P]
   uint8_t *scales;        // [slots][K][P]
   uint32_t *scale_sums;   // [K][P]
@@ -52,6 +53,7 @@ struct F64Args {
   double threshold, mult;
   uint32_t scale_ln;
   uint32_t n, P, K, L;
+  uint32_t cstride;
 };
 hipError_t launch_prune_f64(const F64Args &a, hipStream_t s);
 

@@ -46,8 +46,8 @@ __global__ __launch_bounds__(256) void prune_f64_kernel(F64Args a) {
     const bool rtip = (e.right & SRC_TAG) == SRC_TIP;
     double p0, p1, p2, p3;
     if (rtip) {  // both children are tips: calc_sample_sample
-      const uint32_t sl = a.codes[(size_t)(e.left & SRC_IDX) * a.P + p];
-      const uint32_t sr = a.codes[(size_t)(e.right & SRC_IDX) * a.P + p];
+      const uint32_t sl = a.codes[(size_t)(e.left & SRC_IDX) * a.cstride + p];
+      const uint32_t sr = a.codes[(size_t)(e.right & SRC_IDX) * a.cstride + p];
       p0 = tipv(tl, 0, sl) * tipv(tr, 0, sr);
       p1 = tipv(tl, 1, sl) * tipv(tr, 1, sr);
       p2 = tipv(tl, 2, sl) * tipv(tr, 2, sr);
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void prune_f64_kernel(F64Args a) {
       r3 += tr[12] * y.x; r3 += tr[13] * y.y; r3 += tr[14] * y.z; r3 += tr[15] * y.w;
       double l0, l1, l2, l3;
       if (ltip) {  // calc_sample_partial
-        const uint32_t sl = a.codes[(size_t)(e.left & SRC_IDX) * a.P + p];
+        const uint32_t sl = a.codes[(size_t)(e.left & SRC_IDX) * a.cstride + p];
         l0 = tipv(tl, 0, sl); l1 = tipv(tl, 1, sl);
         l2 = tipv(tl, 2, sl); l3 = tipv(tl, 3, sl);
       } else {  // calc_partial_partial

@@ -160,7 +160,6 @@ def main():
     cpu = None
     err = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
         from oracle import oracle as O
         log("[bench] FP64 CPU oracle (cpu_baseline)")
         otree = O.Tree(L, wl["parents"], wl["children"], wl["heights"])

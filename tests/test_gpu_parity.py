@@ -3,7 +3,7 @@ trees, alignments and models.
 
 Tolerances: FP32-scaled path, per-site |d lnL| <= 2e-6 * max(1, |lnL|) and
 total relative error <= 1e-6 (the stated target); FP64 plain path, per-site
-relative 1e-12.
+relative 2e-15 with host matrices and 1e-11 with the device-built P(t).
 """
 import numpy as np
 import pytest
@@ -107,8 +107,8 @@ def test_device_p_build_matches_oracle(hip_device, precision, model):
         got = calc.likelihoods(c)
         if precision == "f32":
             _assert_f32(got, ref[c])
-        else:
-            np.testing.assert_allclose(got, ref[c], rtol=1e-12, atol=1e-12)
+        else:  # device eigensystem vs host eigensystem: a few ulps in P(t)
+            np.testing.assert_allclose(got, ref[c], rtol=1e-11, atol=1e-11)
     _, total_ref = _mix(ref, w)
     tol = F32_TOTAL if precision == "f32" else 1e-12
     assert abs(calc.total() - total_ref) <= tol * abs(total_ref)
