@@ -400,7 +400,7 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   // tile choice: T sub-tiles of 64 patterns per workgroup, 64*K*T <= 1024
   uint32_t T = 1;
   if (cfg->tile_patterns) T = std::max<uint32_t>(1u, cfg->tile_patterns / 64u);
-  else T = std::max<uint32_t>(1u, 4u / K);
+  else T = std::max<uint32_t>(1u, 16u / K);  // 1024-thread workgroups
   while (64u * K * T > 1024u && T > 1) T--;
   c->T = T;
   c->blocks = (P + 64 * T - 1) / (64 * T);

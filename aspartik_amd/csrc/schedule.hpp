@@ -1,19 +1,22 @@
-// Update-list scheduling for the pruning kernels.
+// Update-list scheduling for the pruning kernels (host, plain C++).
 //
 // Input is what b3's DatedTree::partials_lists() produces (crates/b3/src/
 // parameters/tree.rs:398-461): the dirty internal nodes with children before
 // parents and the root last, and each node's sorted child pair.  The node set
-// is kept; only the order may change, which does not change any node's
-// value.
+// is kept; only the order may change, which does not change any node's value.
 //
-// For the FP32 kernel the list is re-ordered depth first, visiting the child
-// subtree with the larger register need first (Strahler / Sethi-Ullman
-// order).  A child whose value is produced in the same pass is then consumed
-// from a small per-wavefront LDS stack: stack slot b holds the value of the
-// subtree evaluated at depth b, and a subtree needs at most its Strahler
-// number of slots (<= log2(leaves) + 1).  Subtrees deeper than the stack
-// capacity fall back to reading the child's stored partial from HBM, which
-// every node writes anyway.
+// For the FP32 kernel the list is re-ordered depth first (post-order).  A
+// child whose value is produced in the same pass is then consumed without
+// touching HBM:
+//  - the child emitted immediately before its parent (the second child, or
+//    an only child) is still in the lane's registers: SRC_PREV;
+//  - the other (first) child was pushed to a per-wavefront LDS stack slot:
+//    SRC_LDS.  A subtree evaluated at stack base b leaves a pushed result in
+//    slot b and uses slots >= b; the child order at each node is chosen to
+//    minimise the number of slots (a Sethi-Ullman labelling), which is at
+//    most about log2(leaves).
+// Subtrees deeper than the stack capacity fall back to reading the child's
+// stored partial from HBM, which every node writes anyway.
 #pragma once
 
 #include <stdint.h>
@@ -21,7 +24,7 @@
 #include <string>
 #include <vector>
 
-#include "common.hpp"
+#include "entry.hpp"
 
 namespace b3h {
 
@@ -32,8 +35,10 @@ struct Scheduler {
 
   std::vector<int32_t> pos;    // node id -> list index, -1 when absent
   std::vector<int32_t> kid;    // [2n] in-pass child list index or -1
-  std::vector<uint32_t> need;  // [n]
+  std::vector<int32_t> first;  // [n] side (0/1) evaluated first, -1 if < 2 kids
+  std::vector<uint32_t> need;  // slots to evaluate a subtree, result in registers
   std::vector<uint8_t> consumed;
+  std::vector<uint8_t> pushed;
   std::vector<uint32_t> base;
 
   static uint32_t slot_of(uint32_t node, uint32_t L, const uint8_t *editbuf) {
@@ -77,78 +82,93 @@ struct Scheduler {
     }
     entries.resize(n);
     slots_used = 0;
-    auto src_of = [&](uint32_t i, int s) -> uint32_t {
-      const uint32_t ch = children[2 * i + s];
-      if (ch < L) return SRC_TIP | ch;
-      const int32_t k = kid[2 * i + s];
-      if (reorder && k >= 0 && base[k] < cap) return SRC_LDS | base[k];
-      return SRC_HBM | slot_of(ch, L, editbuf);
-    };
     if (!reorder) {
       for (uint32_t i = 0; i < n; i++) {
         Entry &e = entries[i];
         e.out = slot_of(internals[i], L, editbuf);
-        e.left = src_of(i, 0);
-        e.right = src_of(i, 1);
+        for (int s = 0; s < 2; s++) {
+          const uint32_t ch = children[2 * i + s];
+          const uint32_t src = ch < L ? (SRC_TIP | ch) : (SRC_HBM | slot_of(ch, L, editbuf));
+          (s ? e.right : e.left) = src;
+        }
         e.meta = NO_PUSH | (i << 8);
       }
       return true;
     }
-    // register need of every in-pass subtree (list order is post-order)
-    need.assign(n, 1);
+    // Sethi-Ullman labelling over the in-pass forest (list order = post-order)
+    need.assign(n, 0);
+    first.assign(n, -1);
     for (uint32_t i = 0; i < n; i++) {
       const int32_t a = kid[2 * i], b = kid[2 * i + 1];
-      if (a >= 0 && b >= 0)
-        need[i] = need[a] == need[b] ? need[a] + 1 : (need[a] > need[b] ? need[a] : need[b]);
-      else if (a >= 0)
+      if (a >= 0 && b >= 0) {
+        const uint32_t pa = need[a] > 1 ? need[a] : 1, pb = need[b] > 1 ? need[b] : 1;
+        const uint32_t o1 = pa > 1 + need[b] ? pa : 1 + need[b];  // a first
+        const uint32_t o2 = pb > 1 + need[a] ? pb : 1 + need[a];  // b first
+        need[i] = o1 <= o2 ? o1 : o2;
+        first[i] = o1 <= o2 ? 0 : 1;
+      } else if (a >= 0) {
         need[i] = need[a];
-      else if (b >= 0)
+      } else if (b >= 0) {
         need[i] = need[b];
+      }
     }
     base.assign(n, 0);
+    pushed.assign(n, 0);
     struct Frame {
       int32_t v;
       uint32_t base;
       uint8_t state;
+      uint8_t push;
     };
     std::vector<Frame> st;
     st.reserve(64);
     uint32_t out_i = 0;
     for (uint32_t r = 0; r < n; r++) {
       if (consumed[r]) continue;  // not the top of an in-pass subtree
-      st.push_back({(int32_t)r, 0, 0});
+      st.push_back({(int32_t)r, 0, 0, 0});
       while (!st.empty()) {
         const size_t top = st.size() - 1;
         const int32_t v = st[top].v;
         const uint32_t b0 = st[top].base;
-        int32_t first = kid[2 * v], second = kid[2 * v + 1];
-        if (first < 0 || (second >= 0 && need[second] > need[first])) {
-          const int32_t t = first;
-          first = second;
-          second = t;
+        const int32_t a = kid[2 * v], b = kid[2 * v + 1];
+        int32_t k1 = -1, k2 = -1;  // evaluated first (pushed), second (in registers)
+        if (a >= 0 && b >= 0) {
+          k1 = first[v] == 0 ? a : b;
+          k2 = first[v] == 0 ? b : a;
+        } else {
+          k2 = a >= 0 ? a : b;
         }
         if (st[top].state == 0) {
           st[top].state = 1;
-          if (first >= 0) {
-            st.push_back({first, b0, 0});
+          if (k1 >= 0) {
+            st.push_back({k1, b0, 0, 1});
             continue;
           }
         }
         if (st[top].state == 1) {
           st[top].state = 2;
-          if (second >= 0) {
-            st.push_back({second, b0 + 1, 0});
+          if (k2 >= 0) {
+            st.push_back({k2, k1 >= 0 ? b0 + 1 : b0, 0, 0});
             continue;
           }
         }
         base[v] = b0;
+        const bool push = st[top].push && b0 < cap;
+        pushed[v] = push ? 1 : 0;
         Entry &e = entries[out_i++];
         e.out = slot_of(internals[v], L, editbuf);
-        e.left = src_of((uint32_t)v, 0);
-        e.right = src_of((uint32_t)v, 1);
-        const bool pushed = consumed[v] && b0 < cap;
-        e.meta = (pushed ? b0 : NO_PUSH) | ((uint32_t)v << 8);
-        if (pushed && b0 + 1 > slots_used) slots_used = b0 + 1;
+        for (int s = 0; s < 2; s++) {
+          const uint32_t ch = children[2 * v + s];
+          const int32_t k = kid[2 * v + s];
+          uint32_t src;
+          if (ch < L) src = SRC_TIP | ch;
+          else if (k >= 0 && k == k2) src = SRC_PREV;
+          else if (k >= 0 && pushed[k]) src = SRC_LDS | base[k];
+          else src = SRC_HBM | slot_of(ch, L, editbuf);
+          (s ? e.right : e.left) = src;
+        }
+        e.meta = (push ? b0 : NO_PUSH) | ((uint32_t)v << 8);
+        if (push && b0 + 1 > slots_used) slots_used = b0 + 1;
         st.pop_back();
       }
     }

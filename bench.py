@@ -52,15 +52,19 @@ def read_traffic(workload):
 
 def time_evals(calc, lists, times, rates, steps, warmup, sync_each=True):
     internals, children = lists
+    # every evaluation is followed by accept(), as in an MCMC step: b3's FP64
+    # scale counts are not idempotent across proposes without one (cpu4.rs:78-89)
     for _ in range(warmup):
         calc.propose_times(internals, children, times, rates)
         calc.total()
+        calc.accept()
     calc.profile_enable(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         calc.propose_times(internals, children, times, rates)
         if sync_each:
             calc.total()
+        calc.accept()
     total = calc.total()
     dt = time.perf_counter() - t0
     kms, launches = calc.profile_read()
