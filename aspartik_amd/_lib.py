@@ -45,7 +45,8 @@ class Config(C.Structure):
         ("states", C.c_uint32),
         ("scale_ln", C.c_uint32),
         ("tile_patterns", C.c_uint32),
-        ("reserved", C.c_uint32 * 3),
+        ("lane_patterns", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
     ]
 
 

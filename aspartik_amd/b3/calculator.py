@@ -19,17 +19,18 @@ class Calculator:
     """Backend choice + scaling, like b3's ``CalculatorConfig``."""
 
     def __init__(self, precision: int, device: int = 0, tile_patterns: int = 0,
-                 scale_ln: int = 30, states: int = 4):
+                 scale_ln: int = 30, states: int = 4, lane_patterns: int = 0):
         self.precision = int(precision)
         self.device = int(device)
         self.tile_patterns = int(tile_patterns)
+        self.lane_patterns = int(lane_patterns)
         self.scale_ln = int(scale_ln)
         self.states = int(states)
 
     @classmethod
-    def HIP(cls, device: int = 0, tile_patterns: int = 0):
+    def HIP(cls, device: int = 0, tile_patterns: int = 0, lane_patterns: int = 0):
         """FP32 partials with exact power-of-two scaling (the product path)."""
-        return cls(_lib.B3H_F32_SCALED, device, tile_patterns)
+        return cls(_lib.B3H_F32_SCALED, device, tile_patterns, lane_patterns=lane_patterns)
 
     @classmethod
     def HIPPlainF64(cls, device: int = 0):
@@ -69,6 +70,7 @@ class HipCalculator:
         cfg.states = config.states
         cfg.scale_ln = config.scale_ln
         cfg.tile_patterns = config.tile_patterns
+        cfg.lane_patterns = config.lane_patterns
         h = C.c_void_p()
         check(lib().b3h_create(C.byref(cfg), samples, self.L, self.P, self.K, C.byref(h)))
         self._h = h

@@ -44,7 +44,8 @@ struct b3h_calc {
   uint32_t precision = B3H_F32_SCALED;
   uint32_t L = 0, I = 0, P = 0, K = 0, S = 4;
   uint32_t scale_ln = 30;
-  uint32_t T = 1;  // 64-pattern sub-tiles per workgroup
+  uint32_t T = 1;    // 64-lane pattern rows (wavefronts per category) per workgroup
+  uint32_t PPL = 1;  // patterns per lane
   uint32_t cstride = 0;  // device code row stride (P rounded up to 64)
   uint32_t blocks = 0;
 
@@ -270,7 +271,14 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     a.T = c->T;
     a.nslots = std::max<uint32_t>(c->sched.slots_used, 1u);
     if (e0) HIPCHK(hipEventRecord(e0, c->stream));
-    HIPCHK(launch_prune_f32(a, c->blocks, c->stream));
+    // Lists that read children from HBM (partial updates) use the variant
+    // that prefetches them, with one pattern per lane (its registers do not
+    // fit two); lists computed from registers and LDS (full evaluations) use
+    // the configured patterns per lane.
+    const bool hbm = c->sched.hbm_sources * 8 > n;
+    const uint32_t ppl = hbm ? 1u : c->PPL;
+    const uint32_t tp = 64u * c->T * ppl;
+    HIPCHK(launch_prune_f32(a, ppl, hbm, (c->P + tp - 1) / tp, c->stream));
     if (e1) HIPCHK(hipEventRecord(e1, c->stream));
   } else {
     HIPCHK(launch_pbuild<double>(d_entries, n, K, S, mode, d_mats, d_times,
@@ -397,25 +405,35 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   const size_t slots = (size_t)2 * c->I;
   const bool f32 = c->precision == B3H_F32_SCALED;
 
-  // tile choice: T sub-tiles of 64 patterns per workgroup, 64*K*T <= 1024
+  // Tile choice: T wavefront rows per category (64*K*T <= 1024 threads) and
+  // PPL patterns per lane.  Two patterns per lane once the wavefront count
+  // (K*P/64) exceeds what the chip holds at once (~16 per CU x 256 CUs).
   uint32_t T = 1;
   if (cfg->tile_patterns) T = std::max<uint32_t>(1u, cfg->tile_patterns / 64u);
   else T = std::max<uint32_t>(1u, 16u / K);  // 1024-thread workgroups
   while (64u * K * T > 1024u && T > 1) T--;
+  uint32_t PPL = cfg->lane_patterns;
+  if (PPL == 0) PPL = ((uint64_t)K * P / 64 > 4096) ? 2 : 1;
+  if (PPL > 2) {
+    fail(B3H_ERR_ARG, "lane_patterns must be 0, 1 or 2");
+    return bail(B3H_ERR_ARG);
+  }
   c->T = T;
-  c->blocks = (P + 64 * T - 1) / (64 * T);
+  c->PPL = PPL;  // upper bound; partial updates launch with one pattern per lane
+  const uint32_t TP = 64 * T * PPL;
+  c->blocks = (P + 64 * T - 1) / (64 * T);  // the most blocks any launch uses
 
   // LDS stack capacity for this tile shape and the longest update list
   uint32_t cap = kMaxSlots;
-  while (cap > 1 && prune_f32_lds_bytes(K, T, cap, c->I) > kMaxLds) cap--;
-  if (prune_f32_lds_bytes(K, T, cap, c->I) > kMaxLds) {
+  while (cap > 1 && prune_f32_lds_bytes(K, T, cap, c->I, PPL) > kMaxLds) cap--;
+  if (prune_f32_lds_bytes(K, T, cap, c->I, PPL) > kMaxLds) {
     fail(B3H_ERR_UNSUPPORTED, "tree too large for the LDS-staged update list");
     return bail(B3H_ERR_UNSUPPORTED);
   }
   c->sched.capacity = cap;
 
   b3h_status st;
-  c->cstride = (P + 64 * T - 1) / (64 * T) * (64 * T);  // whole tiles, no over-read
+  c->cstride = (P + TP - 1) / TP * TP;  // whole tiles, no over-read
   std::vector<uint8_t> codes((size_t)c->L * c->cstride, (uint8_t)S);
   for (size_t l = 0; l < c->L; l++)
     for (size_t i = 0; i < P; i++)

@@ -34,8 +34,10 @@ struct F32Args {
   uint32_t n, P, K, T, nslots;
   uint32_t cstride;       // code row stride (a multiple of 64 T)
 };
-hipError_t launch_prune_f32(const F32Args &a, uint32_t blocks, hipStream_t s);
-size_t prune_f32_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n);
+hipError_t launch_prune_f32(const F32Args &a, uint32_t ppl, bool hbm, uint32_t blocks,
+                            hipStream_t s);
+size_t prune_f32_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n,
+                           uint32_t ppl);
 hipError_t prepare_prune_f32(int max_lds_bytes);
 
 // ---- prune64.hip --------------------------------------------------------

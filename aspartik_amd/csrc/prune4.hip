@@ -9,9 +9,12 @@
 // sum (likelihood/mod.rs:134-139).
 //
 // Layout and schedule (MI355X):
-//  - a workgroup is K x T wavefronts; wavefront (c, t) owns category c of the
-//    64 patterns of sub-tile t, so one lane = one (pattern, category) and a
-//    partial is one float4 (16 B): a wavefront moves 1 KiB per node, coalesced.
+//  - a workgroup is K x T wavefronts; wavefront (c, t) owns category c of
+//    64 x PPL patterns of sub-tile t.  A lane handles PPL patterns 64 apart,
+//    so one partial is one float4 (16 B) and a wavefront moves PPL KiB per
+//    node in coalesced 1 KiB rows; PPL independent chains per lane hide
+//    latency and halve the wavefront count so a whole evaluation fits on the
+//    chip at once.
 //  - partials [slot][K][P] float4 with an int16 cumulative binary exponent per
 //    (slot, category, pattern).  After every node the four values are scaled
 //    by 2^-e (e = exponent of their maximum), which is exact, so FP32 never
@@ -43,14 +46,7 @@ namespace {
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-constexpr uint32_t CH = 8;          // update-list entries per LDS chunk
-constexpr uint32_t MAT_REGS = 4;   // chunk matrix dwords per thread (T >= 1)
-constexpr uint32_t CODE_REGS = 4;  // chunk code dwords per thread (K >= 1)
-
-struct Src {
-  float4 v;
-  int e;
-};
+constexpr uint32_t CH = 8;  // update-list entries per LDS chunk
 
 __device__ __forceinline__ float col_or_one(float m0, float m1, float m2,
                                             float m3, uint32_t code) {
@@ -62,23 +58,25 @@ __device__ __forceinline__ float col_or_one(float m0, float m1, float m2,
   return r;
 }
 
-__device__ __forceinline__ float4 apply(const float4 *M4, float4 y) {
-  const float4 r0 = M4[0], r1 = M4[1], r2 = M4[2], r3 = M4[3];
+struct Mat {
+  float4 r0, r1, r2, r3;
+};
+
+__device__ __forceinline__ float4 apply(const Mat &M, float4 y) {
   float4 o;
-  o.x = r0.x * y.x + r0.y * y.y + r0.z * y.z + r0.w * y.w;
-  o.y = r1.x * y.x + r1.y * y.y + r1.z * y.z + r1.w * y.w;
-  o.z = r2.x * y.x + r2.y * y.y + r2.z * y.z + r2.w * y.w;
-  o.w = r3.x * y.x + r3.y * y.y + r3.z * y.z + r3.w * y.w;
+  o.x = M.r0.x * y.x + M.r0.y * y.y + M.r0.z * y.z + M.r0.w * y.w;
+  o.y = M.r1.x * y.x + M.r1.y * y.y + M.r1.z * y.z + M.r1.w * y.w;
+  o.z = M.r2.x * y.x + M.r2.y * y.y + M.r2.z * y.z + M.r2.w * y.w;
+  o.w = M.r3.x * y.x + M.r3.y * y.y + M.r3.z * y.z + M.r3.w * y.w;
   return o;
 }
 
-__device__ __forceinline__ float4 tip(const float4 *M4, uint32_t code) {
-  const float4 r0 = M4[0], r1 = M4[1], r2 = M4[2], r3 = M4[3];
+__device__ __forceinline__ float4 tip(const Mat &M, uint32_t code) {
   float4 o;
-  o.x = col_or_one(r0.x, r0.y, r0.z, r0.w, code);
-  o.y = col_or_one(r1.x, r1.y, r1.z, r1.w, code);
-  o.z = col_or_one(r2.x, r2.y, r2.z, r2.w, code);
-  o.w = col_or_one(r3.x, r3.y, r3.z, r3.w, code);
+  o.x = col_or_one(M.r0.x, M.r0.y, M.r0.z, M.r0.w, code);
+  o.y = col_or_one(M.r1.x, M.r1.y, M.r1.z, M.r1.w, code);
+  o.z = col_or_one(M.r2.x, M.r2.y, M.r2.z, M.r2.w, code);
+  o.w = col_or_one(M.r3.x, M.r3.y, M.r3.z, M.r3.w, code);
   return o;
 }
 
@@ -86,21 +84,38 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
+__device__ __forceinline__ Mat load_mat(const float4 *p) {
+  Mat m;
+  m.r0 = p[0];
+  m.r1 = p[1];
+  m.r2 = p[2];
+  m.r3 = p[3];
+  return m;
+}
+
 }  // namespace
 
-size_t prune_f32_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n) {
+size_t prune_f32_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n,
+                           uint32_t ppl) {
   const size_t W = (size_t)K * T;
-  const size_t TP = 64 * (size_t)T;
-  size_t b = (size_t)n * 16;                      // entries
-  b += 2 * ((size_t)CH * 2 * K * 64);             // matrix chunks
-  b += 2 * ((size_t)CH * 2 * TP);                 // code chunks
+  const size_t TP = 64 * (size_t)T * ppl;
+  size_t b = (size_t)n * 16;               // entries
+  b += 2 * ((size_t)CH * 2 * K * 64);      // matrix chunks
+  b += 2 * ((size_t)CH * 2 * TP);          // code chunks
   b = (b + 15) & ~(size_t)15;
-  b += W * nslots * 64 * (16 + 4);                // stacks
-  b += W * 64 * 8 + T * 8 + 16;                   // mix, red, flag
+  b += W * nslots * ppl * 64 * (16 + 4);   // stacks
+  b += W * ppl * 64 * 8 + T * 8 + 16;      // mix, red, flag
   return b;
 }
 
+// HBM: the update list reads children from HBM (partial updates); those are
+// then fetched one node ahead.  Without it (full evaluations) the rare HBM
+// child (LDS stack overflow) is loaded where it is used, and the prefetch
+// registers are not allocated.
+template <int PPL, bool HBM>
 __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
+  constexpr uint32_t MAT_REGS = 4;        // matrix-chunk dwords per thread
+  constexpr uint32_t CODE_REGS = 4 * PPL;  // code-chunk dwords per thread
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t nthr = blockDim.x;
   const uint32_t tid = threadIdx.x;
@@ -109,11 +124,16 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
   const uint32_t K = a.K, T = a.T, n = a.n;
   const uint32_t W = K * T;
   const uint32_t c = w % K, t = w / K;
-  const uint32_t TP = 64u * T;
+  const uint32_t TP = 64u * T * PPL;
   const uint32_t tile0 = blockIdx.x * TP;
-  const uint32_t p = tile0 + t * 64u + lane;
-  const bool active = p < a.P;
-  const uint32_t pp = active ? p : 0u;
+  uint32_t p[PPL], pp[PPL];
+  bool active[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    p[q] = tile0 + t * 64u * PPL + q * 64u + lane;
+    active[q] = p[q] < a.P;
+    pp[q] = active[q] ? p[q] : 0u;
+  }
   const uint32_t ns = a.nslots;
   const size_t KP = (size_t)K * a.P;
   const uint32_t MATC = CH * 2 * K * 16;  // floats per matrix chunk
@@ -128,13 +148,13 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
   unsigned char *code_s = sp;  // [2][CODEC]
   sp += 2 * (size_t)CODEC;
   sp = smem + ((sp - smem + 15) & ~(size_t)15);
-  float4 *stk_all = reinterpret_cast<float4 *>(sp);
-  int *stke_all = reinterpret_cast<int *>(stk_all + (size_t)W * ns * 64);
-  double *mix = reinterpret_cast<double *>(stke_all + (size_t)W * ns * 64);
-  double *red = mix + (size_t)W * 64;
+  float4 *stk_all = reinterpret_cast<float4 *>(sp);  // [W][ns][PPL][64]
+  int *stke_all = reinterpret_cast<int *>(stk_all + (size_t)W * ns * PPL * 64);
+  double *mix = reinterpret_cast<double *>(stke_all + (size_t)W * ns * PPL * 64);
+  double *red = mix + (size_t)W * PPL * 64;
   int *flag = reinterpret_cast<int *>(red + T);
-  float4 *stk = stk_all + (size_t)w * ns * 64;
-  int *stke = stke_all + (size_t)w * ns * 64;
+  float4 *stk = stk_all + (size_t)w * ns * PPL * 64;
+  int *stke = stke_all + (size_t)w * ns * PPL * 64;
 
   // ---- stage the update list ----
   const uint4 *g_ent = reinterpret_cast<const uint4 *>(a.entries);
@@ -142,11 +162,11 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
   __syncthreads();
 
   // Chunk fetch into registers / store into an LDS buffer.  Code rows are
-  // padded to a multiple of 64 patterns on the device (cstride), so a tile's
-  // codes of one leaf are TP/4 aligned dwords.
+  // padded to whole tiles on the device (cstride), so a tile's codes of one
+  // leaf are TP/4 aligned dwords.
   float mreg[MAT_REGS];
   uint32_t creg[CODE_REGS];
-  const uint32_t TPW = TP / 4;  // code dwords per tip row
+  const uint32_t TPW = TP / 4;
 #define B3H_FETCH(ck)                                                          \
   do {                                                                         \
     const uint32_t base_ = (ck) * CH;                                          \
@@ -198,26 +218,41 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
   __syncthreads();
 
   // HBM children of the next node, fetched one node ahead
-  float4 hl = make_float4(0.f, 0.f, 0.f, 0.f), hr = hl;
-  int hel = 0, her = 0;
-  auto fetch_hbm = [&](uint32_t i, float4 &vl, int &el, float4 &vr, int &er) {
-    const uint4 e = ent_s[i];
-    const uint32_t lsrc = uni(e.y), rsrc = uni(e.z);
-    if ((lsrc & SRC_TAG) == SRC_HBM) {
-      const size_t o = (size_t)(lsrc & SRC_IDX) * KP + (size_t)c * a.P + pp;
-      vl = a.partials[o];
-      el = a.exps[o];
-    }
-    if ((rsrc & SRC_TAG) == SRC_HBM) {
-      const size_t o = (size_t)(rsrc & SRC_IDX) * KP + (size_t)c * a.P + pp;
-      vr = a.partials[o];
-      er = a.exps[o];
-    }
-  };
-  fetch_hbm(0, hl, hel, hr, her);
+  float4 hl[PPL], hr[PPL];
+  int hel[PPL], her[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    hl[q] = hr[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    hel[q] = her[q] = 0;
+  }
+#define B3H_FETCH_HBM(i)                                                       \
+  do {                                                                         \
+    const uint4 e_ = ent_s[(i)];                                               \
+    const uint32_t ls_ = uni(e_.y), rs_ = uni(e_.z);                           \
+    if ((ls_ & SRC_TAG) == SRC_HBM) {                                          \
+      const size_t o_ = (size_t)(ls_ & SRC_IDX) * KP + (size_t)c * a.P;        \
+      _Pragma("unroll") for (int q = 0; q < PPL; q++) {                        \
+        hl[q] = a.partials[o_ + pp[q]];                                        \
+        hel[q] = a.exps[o_ + pp[q]];                                           \
+      }                                                                        \
+    }                                                                          \
+    if ((rs_ & SRC_TAG) == SRC_HBM) {                                          \
+      const size_t o_ = (size_t)(rs_ & SRC_IDX) * KP + (size_t)c * a.P;        \
+      _Pragma("unroll") for (int q = 0; q < PPL; q++) {                        \
+        hr[q] = a.partials[o_ + pp[q]];                                        \
+        her[q] = a.exps[o_ + pp[q]];                                           \
+      }                                                                        \
+    }                                                                          \
+  } while (0)
+  if (HBM) B3H_FETCH_HBM(0u);
 
-  float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
-  int E = 0;
+  float4 x[PPL];
+  int E[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    x[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    E[q] = 0;
+  }
   for (uint32_t ck = 0; ck < nchunks; ck++) {
     const uint32_t buf = ck & 1u;
     const bool more = ck + 1 < nchunks;
@@ -231,70 +266,99 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
       const uint4 e4 = ent_s[i];
       const uint32_t out = uni(e4.x), lsrc = uni(e4.y), rsrc = uni(e4.z),
                      meta = uni(e4.w);
-      const float4 cl = hl, cr = hr;
-      const int cel = hel, cer = her;
-      if (i + 1 < n) fetch_hbm(i + 1, hl, hel, hr, her);
-      const float4 *ML = ms + ((size_t)(j * 2 + 0) * K + c) * 4;
-      const float4 *MR = ms + ((size_t)(j * 2 + 1) * K + c) * 4;
-      Src l, r;
+      float4 cl[PPL], cr[PPL];
+      int cel[PPL], cer[PPL];
+      if (HBM) {
+#pragma unroll
+        for (int q = 0; q < PPL; q++) {
+          cl[q] = hl[q];
+          cr[q] = hr[q];
+          cel[q] = hel[q];
+          cer[q] = her[q];
+        }
+        if (i + 1 < n) B3H_FETCH_HBM(i + 1);
+      }
+      const Mat ML = load_mat(ms + ((size_t)(j * 2 + 0) * K + c) * 4);
+      const Mat MR = load_mat(ms + ((size_t)(j * 2 + 1) * K + c) * 4);
+      float4 lv[PPL], rv[PPL];
+      int le[PPL], re[PPL];
       {
         const uint32_t tag = lsrc & SRC_TAG, idx = lsrc & SRC_IDX;
-        if (tag == SRC_TIP) {
-          l.v = tip(ML, cs[(j * 2 + 0) * TP + t * 64 + lane]);
-          l.e = 0;
-        } else if (tag == SRC_PREV) {
-          l.v = apply(ML, x);
-          l.e = E;
-        } else if (tag == SRC_LDS) {
-          l.v = apply(ML, stk[idx * 64 + lane]);
-          l.e = stke[idx * 64 + lane];
-        } else {
-          l.v = apply(ML, cl);
-          l.e = cel;
+#pragma unroll
+        for (int q = 0; q < PPL; q++) {
+          if (tag == SRC_TIP) {
+            lv[q] = tip(ML, cs[(j * 2 + 0) * TP + t * 64 * PPL + q * 64 + lane]);
+            le[q] = 0;
+          } else if (tag == SRC_PREV) {
+            lv[q] = apply(ML, x[q]);
+            le[q] = E[q];
+          } else if (tag == SRC_LDS) {
+            lv[q] = apply(ML, stk[(idx * PPL + q) * 64 + lane]);
+            le[q] = stke[(idx * PPL + q) * 64 + lane];
+          } else if (HBM) {
+            lv[q] = apply(ML, cl[q]);
+            le[q] = cel[q];
+          } else {
+            const size_t o = (size_t)idx * KP + (size_t)c * a.P + pp[q];
+            lv[q] = apply(ML, a.partials[o]);
+            le[q] = a.exps[o];
+          }
         }
       }
       {
         const uint32_t tag = rsrc & SRC_TAG, idx = rsrc & SRC_IDX;
-        if (tag == SRC_TIP) {
-          r.v = tip(MR, cs[(j * 2 + 1) * TP + t * 64 + lane]);
-          r.e = 0;
-        } else if (tag == SRC_PREV) {
-          r.v = apply(MR, x);
-          r.e = E;
-        } else if (tag == SRC_LDS) {
-          r.v = apply(MR, stk[idx * 64 + lane]);
-          r.e = stke[idx * 64 + lane];
-        } else {
-          r.v = apply(MR, cr);
-          r.e = cer;
+#pragma unroll
+        for (int q = 0; q < PPL; q++) {
+          if (tag == SRC_TIP) {
+            rv[q] = tip(MR, cs[(j * 2 + 1) * TP + t * 64 * PPL + q * 64 + lane]);
+            re[q] = 0;
+          } else if (tag == SRC_PREV) {
+            rv[q] = apply(MR, x[q]);
+            re[q] = E[q];
+          } else if (tag == SRC_LDS) {
+            rv[q] = apply(MR, stk[(idx * PPL + q) * 64 + lane]);
+            re[q] = stke[(idx * PPL + q) * 64 + lane];
+          } else if (HBM) {
+            rv[q] = apply(MR, cr[q]);
+            re[q] = cer[q];
+          } else {
+            const size_t o = (size_t)idx * KP + (size_t)c * a.P + pp[q];
+            rv[q] = apply(MR, a.partials[o]);
+            re[q] = a.exps[o];
+          }
         }
       }
-      x.x = l.v.x * r.v.x;
-      x.y = l.v.y * r.v.y;
-      x.z = l.v.z * r.v.z;
-      x.w = l.v.w * r.v.w;
-      const float mx = fmaxf(fmaxf(x.x, x.y), fmaxf(x.z, x.w));
-      const int ex = (mx > 0.0f) ? __builtin_amdgcn_frexp_expf(mx) : 0;
-      x.x = __builtin_amdgcn_ldexpf(x.x, -ex);
-      x.y = __builtin_amdgcn_ldexpf(x.y, -ex);
-      x.z = __builtin_amdgcn_ldexpf(x.z, -ex);
-      x.w = __builtin_amdgcn_ldexpf(x.w, -ex);
-      E = l.e + r.e + ex;
-      if (active) {
-        const size_t o = (size_t)out * KP + (size_t)c * a.P + p;
-#if B3H_NT_STORE
-        const f4v xv = {x.x, x.y, x.z, x.w};
-        __builtin_nontemporal_store(xv, reinterpret_cast<f4v *>(&a.partials[o]));
-        __builtin_nontemporal_store((int16_t)E, &a.exps[o]);
-#else
-        a.partials[o] = x;
-        a.exps[o] = (int16_t)E;
-#endif
-      }
       const uint32_t push = meta & 0xFFu;
-      if (push != NO_PUSH) {
-        stk[push * 64 + lane] = x;
-        stke[push * 64 + lane] = E;
+#pragma unroll
+      for (int q = 0; q < PPL; q++) {
+        float4 v;
+        v.x = lv[q].x * rv[q].x;
+        v.y = lv[q].y * rv[q].y;
+        v.z = lv[q].z * rv[q].z;
+        v.w = lv[q].w * rv[q].w;
+        const float mx = fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w));
+        const int ex = (mx > 0.0f) ? __builtin_amdgcn_frexp_expf(mx) : 0;
+        v.x = __builtin_amdgcn_ldexpf(v.x, -ex);
+        v.y = __builtin_amdgcn_ldexpf(v.y, -ex);
+        v.z = __builtin_amdgcn_ldexpf(v.z, -ex);
+        v.w = __builtin_amdgcn_ldexpf(v.w, -ex);
+        x[q] = v;
+        E[q] = le[q] + re[q] + ex;
+        if (active[q]) {
+          const size_t o = (size_t)out * KP + (size_t)c * a.P + p[q];
+#if B3H_NT_STORE
+          const f4v xv = {v.x, v.y, v.z, v.w};
+          __builtin_nontemporal_store(xv, reinterpret_cast<f4v *>(&a.partials[o]));
+          __builtin_nontemporal_store((int16_t)E[q], &a.exps[o]);
+#else
+          a.partials[o] = v;
+          a.exps[o] = (int16_t)E[q];
+#endif
+        }
+        if (push != NO_PUSH) {
+          stk[(push * PPL + q) * 64 + lane] = v;
+          stke[(push * PPL + q) * 64 + lane] = E[q];
+        }
       }
     }
     if (more) B3H_STORE(buf ^ 1u, ck + 1);
@@ -302,29 +366,37 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
   }
 
   // root: ln(pi . x) + E ln 2 in FP64 (cpu4.rs:277-284, cpu4.rs:69-73)
-  double site = -INFINITY;
-  if (active) {
-    double v = (double)x.x * a.pi0 + (double)x.y * a.pi1;
-    v = v + (double)x.z * a.pi2;
-    v = v + (double)x.w * a.pi3;
-    site = log(v) + (double)E * LN2;
-    a.site[(size_t)c * a.P + p] = site;
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    double site = -INFINITY;
+    if (active[q]) {
+      double v = (double)x[q].x * a.pi0 + (double)x[q].y * a.pi1;
+      v = v + (double)x[q].z * a.pi2;
+      v = v + (double)x[q].w * a.pi3;
+      site = log(v) + (double)E[q] * LN2;
+      a.site[(size_t)c * a.P + p[q]] = site;
+    }
+    mix[(w * PPL + q) * 64 + lane] = site;
   }
-  mix[w * 64 + lane] = site;
   __syncthreads();
   double wv = 0.0;
   if (c == 0) {
-    double mxv = -INFINITY;
-    for (uint32_t k = 0; k < K; k++) mxv = fmax(mxv, mix[(t * K + k) * 64 + lane]);
-    double mixed = -INFINITY;
-    if (mxv != -INFINITY) {
-      double se = 0.0;
-      for (uint32_t k = 0; k < K; k++) se += exp(mix[(t * K + k) * 64 + lane] - mxv);
-      mixed = mxv + log(se) + a.ln_inv_k;
-    }
-    if (active) {
-      a.mixed[p] = mixed;
-      wv = mixed * (double)a.weights[p];
+#pragma unroll
+    for (int q = 0; q < PPL; q++) {
+      double mxv = -INFINITY;
+      for (uint32_t k = 0; k < K; k++)
+        mxv = fmax(mxv, mix[((t * K + k) * PPL + q) * 64 + lane]);
+      double mixed = -INFINITY;
+      if (mxv != -INFINITY) {
+        double se = 0.0;
+        for (uint32_t k = 0; k < K; k++)
+          se += exp(mix[((t * K + k) * PPL + q) * 64 + lane] - mxv);
+        mixed = mxv + log(se) + a.ln_inv_k;
+      }
+      if (active[q]) {
+        a.mixed[p[q]] = mixed;
+        wv += mixed * (double)a.weights[p[q]];
+      }
     }
     wv = wave_sum(wv);
     if (lane == 0) red[t] = wv;
@@ -336,19 +408,34 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
   last_block_total(bsum, a.blocksum, a.counter, a.total, flag);
 #undef B3H_FETCH
 #undef B3H_STORE
+#undef B3H_FETCH_HBM
 }
 
 hipError_t prepare_prune_f32(int max_lds_bytes) {
-  return hipFuncSetAttribute(reinterpret_cast<const void *>(&prune_f32_kernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             max_lds_bytes);
+  const void *fns[] = {reinterpret_cast<const void *>(&prune_f32_kernel<1, false>),
+                       reinterpret_cast<const void *>(&prune_f32_kernel<1, true>),
+                       reinterpret_cast<const void *>(&prune_f32_kernel<2, false>),
+                       reinterpret_cast<const void *>(&prune_f32_kernel<2, true>)};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       max_lds_bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
-hipError_t launch_prune_f32(const F32Args &a, uint32_t blocks, hipStream_t s) {
-  const size_t lds = prune_f32_lds_bytes(a.K, a.T, a.nslots, a.n);
-  const uint32_t threads = 64u * a.K * a.T;
-  hipLaunchKernelGGL(prune_f32_kernel, dim3(blocks), dim3(threads),
-                     (uint32_t)lds, s, a);
+hipError_t launch_prune_f32(const F32Args &a, uint32_t ppl, bool hbm, uint32_t blocks,
+                            hipStream_t s) {
+  const size_t lds = prune_f32_lds_bytes(a.K, a.T, a.nslots, a.n, ppl);
+  const dim3 g(blocks), b(64u * a.K * a.T);
+  const uint32_t l = (uint32_t)lds;
+  if (ppl == 2) {
+    if (hbm) hipLaunchKernelGGL((prune_f32_kernel<2, true>), g, b, l, s, a);
+    else hipLaunchKernelGGL((prune_f32_kernel<2, false>), g, b, l, s, a);
+  } else {
+    if (hbm) hipLaunchKernelGGL((prune_f32_kernel<1, true>), g, b, l, s, a);
+    else hipLaunchKernelGGL((prune_f32_kernel<1, false>), g, b, l, s, a);
+  }
   return hipGetLastError();
 }
 

@@ -31,6 +31,7 @@ namespace b3h {
 struct Scheduler {
   uint32_t capacity = 16;  // LDS stack slots per wavefront
   uint32_t slots_used = 0;
+  uint32_t hbm_sources = 0;  // children read from HBM
   std::vector<Entry> entries;
 
   std::vector<int32_t> pos;    // node id -> list index, -1 when absent
@@ -82,6 +83,7 @@ struct Scheduler {
     }
     entries.resize(n);
     slots_used = 0;
+    hbm_sources = 0;
     if (!reorder) {
       for (uint32_t i = 0; i < n; i++) {
         Entry &e = entries[i];
@@ -89,6 +91,7 @@ struct Scheduler {
         for (int s = 0; s < 2; s++) {
           const uint32_t ch = children[2 * i + s];
           const uint32_t src = ch < L ? (SRC_TIP | ch) : (SRC_HBM | slot_of(ch, L, editbuf));
+          hbm_sources += ch >= L;
           (s ? e.right : e.left) = src;
         }
         e.meta = NO_PUSH | (i << 8);
@@ -165,6 +168,7 @@ struct Scheduler {
           else if (k >= 0 && k == k2) src = SRC_PREV;
           else if (k >= 0 && pushed[k]) src = SRC_LDS | base[k];
           else src = SRC_HBM | slot_of(ch, L, editbuf);
+          hbm_sources += (src & SRC_TAG) == SRC_HBM;
           (s ? e.right : e.left) = src;
         }
         e.meta = (push ? b0 : NO_PUSH) | ((uint32_t)v << 8);

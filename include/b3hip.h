@@ -62,8 +62,9 @@ typedef struct b3h_config {
                               F64_PLAIN path scales by e^scale_ln.  The FP32
                               and FP16 paths scale by exact powers of two and
                               only check it is nonzero.                       */
-  uint32_t tile_patterns;  /* patterns per workgroup tile; 0 = tuned default  */
-  uint32_t reserved[3];
+  uint32_t tile_patterns;  /* 64-pattern rows per workgroup x 64; 0 = tuned  */
+  uint32_t lane_patterns;  /* patterns per lane (1 or 2); 0 = tuned           */
+  uint32_t reserved[2];
 } b3h_config;
 
 typedef struct b3h_calc b3h_calc;

@@ -240,8 +240,8 @@ def test_deep_tree_underflow_scaling(hip_device):
     assert abs(calc.total() - total_ref) <= F32_TOTAL * abs(total_ref)
 
 
-@pytest.mark.parametrize("tile", [64, 128, 256])
-def test_tile_shapes_are_bit_identical(hip_device, tile):
+@pytest.mark.parametrize("tile,lanes", [(64, 1), (128, 1), (256, 1), (64, 2), (256, 2), (512, 2)])
+def test_tile_shapes_are_bit_identical(hip_device, tile, lanes):
     L, P, K = 33, 5000, 2
     rng, tree, om, pm, codes, w = _setup(L, P, K, seed=5)
     t = O.Tree(L, *tree)
@@ -251,7 +251,7 @@ def test_tile_shapes_are_bit_identical(hip_device, tile):
     mats = model_matrices(om, t, children, rates)
     base = Calculator.HIP().make(codes, P, K)
     base.propose(internals, children, mats, om.frequencies())
-    other = Calculator.HIP(tile_patterns=tile).make(codes, P, K)
+    other = Calculator.HIP(tile_patterns=tile, lane_patterns=lanes).make(codes, P, K)
     other.propose(internals, children, mats, om.frequencies())
     for c in range(K):
         np.testing.assert_array_equal(other.likelihoods(c), base.likelihoods(c))
