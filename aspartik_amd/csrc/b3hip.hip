@@ -278,7 +278,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     const bool hbm = c->sched.hbm_sources * 8 > n;
     const uint32_t ppl = hbm ? 1u : c->PPL;
     const uint32_t tp = 64u * c->T * ppl;
-    HIPCHK(launch_prune_f32(a, ppl, hbm, (c->P + tp - 1) / tp, c->stream));
+    HIPCHK(launch_prune_f32(a, ppl, hbm, (c->P + tp - 1) / tp, c->stream));  // hbm => ppl 1
     if (e1) HIPCHK(hipEventRecord(e1, c->stream));
   } else {
     HIPCHK(launch_pbuild<double>(d_entries, n, K, S, mode, d_mats, d_times,

@@ -62,12 +62,18 @@ struct Mat {
   float4 r0, r1, r2, r3;
 };
 
+// Explicit fused multiply-adds in a fixed order, so every kernel variant
+// (tile shape, patterns per lane) rounds identically.
+__device__ __forceinline__ float dot4(float4 m, float4 y) {
+  return fmaf(m.w, y.w, fmaf(m.z, y.z, fmaf(m.y, y.y, m.x * y.x)));
+}
+
 __device__ __forceinline__ float4 apply(const Mat &M, float4 y) {
   float4 o;
-  o.x = M.r0.x * y.x + M.r0.y * y.y + M.r0.z * y.z + M.r0.w * y.w;
-  o.y = M.r1.x * y.x + M.r1.y * y.y + M.r1.z * y.z + M.r1.w * y.w;
-  o.z = M.r2.x * y.x + M.r2.y * y.y + M.r2.z * y.z + M.r2.w * y.w;
-  o.w = M.r3.x * y.x + M.r3.y * y.y + M.r3.z * y.z + M.r3.w * y.w;
+  o.x = dot4(M.r0, y);
+  o.y = dot4(M.r1, y);
+  o.z = dot4(M.r2, y);
+  o.w = dot4(M.r3, y);
   return o;
 }
 
@@ -370,9 +376,10 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
   for (int q = 0; q < PPL; q++) {
     double site = -INFINITY;
     if (active[q]) {
-      double v = (double)x[q].x * a.pi0 + (double)x[q].y * a.pi1;
-      v = v + (double)x[q].z * a.pi2;
-      v = v + (double)x[q].w * a.pi3;
+      double v = (double)x[q].x * a.pi0;
+      v = fma((double)x[q].y, a.pi1, v);
+      v = fma((double)x[q].z, a.pi2, v);
+      v = fma((double)x[q].w, a.pi3, v);
       site = log(v) + (double)E[q] * LN2;
       a.site[(size_t)c * a.P + p[q]] = site;
     }
@@ -390,12 +397,12 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
       if (mxv != -INFINITY) {
         double se = 0.0;
         for (uint32_t k = 0; k < K; k++)
-          se += exp(mix[((t * K + k) * PPL + q) * 64 + lane] - mxv);
-        mixed = mxv + log(se) + a.ln_inv_k;
+          se = se + exp(mix[((t * K + k) * PPL + q) * 64 + lane] - mxv);
+        mixed = (mxv + log(se)) + a.ln_inv_k;
       }
       if (active[q]) {
         a.mixed[p[q]] = mixed;
-        wv += mixed * (double)a.weights[p[q]];
+        wv = fma(mixed, (double)a.weights[p[q]], wv);
       }
     }
     wv = wave_sum(wv);
@@ -414,8 +421,7 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
 hipError_t prepare_prune_f32(int max_lds_bytes) {
   const void *fns[] = {reinterpret_cast<const void *>(&prune_f32_kernel<1, false>),
                        reinterpret_cast<const void *>(&prune_f32_kernel<1, true>),
-                       reinterpret_cast<const void *>(&prune_f32_kernel<2, false>),
-                       reinterpret_cast<const void *>(&prune_f32_kernel<2, true>)};
+                       reinterpret_cast<const void *>(&prune_f32_kernel<2, false>)};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        max_lds_bytes);
@@ -429,9 +435,8 @@ hipError_t launch_prune_f32(const F32Args &a, uint32_t ppl, bool hbm, uint32_t b
   const size_t lds = prune_f32_lds_bytes(a.K, a.T, a.nslots, a.n, ppl);
   const dim3 g(blocks), b(64u * a.K * a.T);
   const uint32_t l = (uint32_t)lds;
-  if (ppl == 2) {
-    if (hbm) hipLaunchKernelGGL((prune_f32_kernel<2, true>), g, b, l, s, a);
-    else hipLaunchKernelGGL((prune_f32_kernel<2, false>), g, b, l, s, a);
+  if (ppl == 2 && !hbm) {  // the HBM-prefetch variant is built with one pattern per lane
+    hipLaunchKernelGGL((prune_f32_kernel<2, false>), g, b, l, s, a);
   } else {
     if (hbm) hipLaunchKernelGGL((prune_f32_kernel<1, true>), g, b, l, s, a);
     else hipLaunchKernelGGL((prune_f32_kernel<1, false>), g, b, l, s, a);
