@@ -380,7 +380,7 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
       v = fma((double)x[q].y, a.pi1, v);
       v = fma((double)x[q].z, a.pi2, v);
       v = fma((double)x[q].w, a.pi3, v);
-      site = log(v) + (double)E[q] * LN2;
+      site = fma((double)E[q], LN2, log(v));
       a.site[(size_t)c * a.P + p[q]] = site;
     }
     mix[(w * PPL + q) * 64 + lane] = site;
