@@ -46,7 +46,7 @@ namespace {
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-constexpr uint32_t CH = 8;  // update-list entries per LDS chunk
+constexpr uint32_t CH = 4;  // update-list entries per LDS chunk
 
 __device__ __forceinline__ float col_or_one(float m0, float m1, float m2,
                                             float m3, uint32_t code) {
@@ -109,8 +109,11 @@ size_t prune_f32_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n,
   b += 2 * ((size_t)CH * 2 * K * 64);      // matrix chunks
   b += 2 * ((size_t)CH * 2 * TP);          // code chunks
   b = (b + 15) & ~(size_t)15;
-  b += W * nslots * ppl * 64 * (16 + 4);   // stacks
-  b += W * ppl * 64 * 8 + T * 8 + 16;      // mix, red, flag
+  // the stacks (float4 + int16 per lane and slot) are dead once the node loop
+  // ends, and the category-mix buffers reuse their space
+  const size_t stacks = W * nslots * ppl * 64 * (16 + 2);
+  const size_t mixbuf = W * ppl * 64 * 8 + T * 8 + 16;
+  b += stacks > mixbuf ? stacks : mixbuf;
   return b;
 }
 
@@ -155,12 +158,12 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
   sp += 2 * (size_t)CODEC;
   sp = smem + ((sp - smem + 15) & ~(size_t)15);
   float4 *stk_all = reinterpret_cast<float4 *>(sp);  // [W][ns][PPL][64]
-  int *stke_all = reinterpret_cast<int *>(stk_all + (size_t)W * ns * PPL * 64);
-  double *mix = reinterpret_cast<double *>(stke_all + (size_t)W * ns * PPL * 64);
+  int16_t *stke_all = reinterpret_cast<int16_t *>(stk_all + (size_t)W * ns * PPL * 64);
+  double *mix = reinterpret_cast<double *>(sp);  // after the node loop only
   double *red = mix + (size_t)W * PPL * 64;
   int *flag = reinterpret_cast<int *>(red + T);
   float4 *stk = stk_all + (size_t)w * ns * PPL * 64;
-  int *stke = stke_all + (size_t)w * ns * PPL * 64;
+  int16_t *stke = stke_all + (size_t)w * ns * PPL * 64;
 
   // ---- stage the update list ----
   const uint4 *g_ent = reinterpret_cast<const uint4 *>(a.entries);
@@ -363,7 +366,7 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
         }
         if (push != NO_PUSH) {
           stk[(push * PPL + q) * 64 + lane] = v;
-          stke[(push * PPL + q) * 64 + lane] = E[q];
+          stke[(push * PPL + q) * 64 + lane] = (int16_t)E[q];
         }
       }
     }
@@ -371,7 +374,8 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
     __syncthreads();
   }
 
-  // root: ln(pi . x) + E ln 2 in FP64 (cpu4.rs:277-284, cpu4.rs:69-73)
+  // root: ln(pi . x) + E ln 2 in FP64 (cpu4.rs:277-284, cpu4.rs:69-73); the
+  // last node-loop barrier separates the stacks from the mix buffers
 #pragma unroll
   for (int q = 0; q < PPL; q++) {
     double site = -INFINITY;
