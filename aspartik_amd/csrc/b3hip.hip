@@ -412,7 +412,7 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   if (PPL == 0) PPL = ((uint64_t)K * P / 64 > 4096) ? 2 : 1;
   uint32_t T = 1;
   if (cfg->tile_patterns) T = std::max<uint32_t>(1u, cfg->tile_patterns / 64u);
-  else if (PPL == 1) T = std::max<uint32_t>(1u, 16u / K);  // 1024-thread workgroups
+  else T = std::max<uint32_t>(1u, 16u / K);  // 1024-thread workgroups (measured best)
   while (64u * K * T > 1024u && T > 1) T--;
   if (PPL > 2) {
     fail(B3H_ERR_ARG, "lane_patterns must be 0, 1 or 2");
