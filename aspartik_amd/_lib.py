@@ -50,6 +50,28 @@ class Config(C.Structure):
     ]
 
 
+class Instance(C.Structure):
+    _fields_ = [
+        ("samples", C.c_void_p),
+        ("num_leaves", C.c_uint32),
+        ("num_patterns", C.c_uint32),
+        ("weights", C.c_void_p),
+    ]
+
+
+class Update(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32),
+        ("internals", C.c_void_p),
+        ("children", C.c_void_p),
+        ("edge_times", C.c_void_p),
+        ("category_rates", C.c_void_p),
+    ]
+
+
+B3H_BATCH_SINGLE_SLOT = 1
+B3H_BATCH_GRAPH = 1
+
 u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -78,6 +100,18 @@ SIGNATURES = {
     "b3h_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "b3h_device_bytes": (C.c_uint64, [C.c_void_p]),
     "b3h_gamma_rates": (C.c_int, [C.c_double, C.c_uint32, f64p]),
+    "b3h_batch_create": (C.c_int, [C.POINTER(Config), C.c_uint32, C.POINTER(Instance), C.c_uint32,
+                                   C.c_uint32, C.POINTER(C.c_void_p)]),
+    "b3h_batch_destroy": (C.c_int, [C.c_void_p]),
+    "b3h_batch_set_model": (C.c_int, [C.c_void_p, C.c_uint32, f64p, f64p, C.c_uint32]),
+    "b3h_batch_propose_times": (C.c_int, [C.c_void_p, C.POINTER(Update), C.c_uint32]),
+    "b3h_batch_totals": (C.c_int, [C.c_void_p, f64p]),
+    "b3h_batch_likelihoods": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, f64p]),
+    "b3h_batch_accept": (C.c_int, [C.c_void_p]),
+    "b3h_batch_reject": (C.c_int, [C.c_void_p]),
+    "b3h_batch_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "b3h_batch_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "b3h_batch_device_bytes": (C.c_uint64, [C.c_void_p]),
 }
 
 

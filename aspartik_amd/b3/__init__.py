@@ -1,12 +1,13 @@
 """b3-facing surface of aspartik_amd (mirrors aspartik.b3's calculator,
 tree, substitution and likelihood objects that drive the pruning path)."""
+from .batch import Batch
 from .calculator import Calculator, HipCalculator, gamma_rates
 from .likelihood import DNALikelihood, GammaLikelihood, deduplicate
 from .substitution import GTR, GY94, HKY, JC, K80, SENSE_CODONS
 from .tree import TOMBSTONE, DatedTree
 
 __all__ = [
-    "Calculator", "HipCalculator", "gamma_rates", "DNALikelihood", "GammaLikelihood",
+    "Batch", "Calculator", "HipCalculator", "gamma_rates", "DNALikelihood", "GammaLikelihood",
     "deduplicate", "GTR", "GY94", "HKY", "JC", "K80", "SENSE_CODONS", "TOMBSTONE",
     "DatedTree",
 ]

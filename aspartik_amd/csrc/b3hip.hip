@@ -24,6 +24,16 @@ b3h_status fail(b3h_status s, const std::string &msg) {
   return s;
 }
 
+}  // namespace
+
+namespace b3h {
+// shared with the other host translation units (batch.hip)
+b3h_status host_fail(b3h_status s, const std::string &msg) { return fail(s, msg); }
+uint8_t host_remap_code(uint8_t v, uint32_t S);
+}  // namespace b3h
+
+namespace {
+
 #define HIPCHK(expr)                                                   \
   do {                                                                 \
     hipError_t e_ = (expr);                                            \
@@ -125,7 +135,11 @@ void release(b3h_calc *c) {
 // Tip codes as the kernels read them: 4 states use cpu4.rs:117-125's remap
 // {1,2,4,8} -> {0,1,2,3}, everything else 4 (unknown); S > 4 keeps indices,
 // clamping unknowns to S.
-uint8_t remap_code(uint8_t v, uint32_t S) {
+uint8_t remap_code(uint8_t v, uint32_t S) { return b3h::host_remap_code(v, S); }
+
+}  // namespace
+
+uint8_t b3h::host_remap_code(uint8_t v, uint32_t S) {
   if (S == 4) {
     switch (v) {
       case 1: return 0;
@@ -137,6 +151,8 @@ uint8_t remap_code(uint8_t v, uint32_t S) {
   }
   return v < S ? v : (uint8_t)S;
 }
+
+namespace {
 
 b3h_status check_handle(const b3h_calc *c) {
   if (!c) return fail(B3H_ERR_ARG, "null calculator handle");
@@ -278,6 +294,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     const bool hbm = c->sched.hbm_sources * 8 > n;
     const uint32_t ppl = hbm ? 1u : c->PPL;
     const uint32_t tp = 64u * c->T * ppl;
+    a.ntiles = (c->P + tp - 1) / tp;
     HIPCHK(launch_prune_f32(a, ppl, hbm, (c->P + tp - 1) / tp, c->stream));  // hbm => ppl 1
     if (e1) HIPCHK(hipEventRecord(e1, c->stream));
   } else {

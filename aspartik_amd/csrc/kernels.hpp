@@ -13,6 +13,18 @@ hipError_t launch_pbuild(const Entry *entries, uint32_t n, uint32_t K,
                          const double *times, const double *rates,
                          const DevModel *model, T *out, hipStream_t s);
 
+struct PbuildDesc {
+  const Entry *entries;
+  const double *times;
+  const double *rates;
+  const DevModel *model;
+  float *out;
+  uint32_t n;
+  uint32_t pad;
+};
+hipError_t launch_pbuild_batch(const PbuildDesc *desc, uint32_t count, uint32_t max_n,
+                               uint32_t K, uint32_t S, hipStream_t s);
+
 // ---- prune4.hip ---------------------------------------------------------
 // FP32 partials, 4 states, K categories per workgroup (one wavefront per
 // category and 64-pattern sub-tile), LDS stack for in-pass children,
@@ -33,11 +45,18 @@ struct F32Args {
   double ln_inv_k;
   uint32_t n, P, K, T, nslots;
   uint32_t cstride;       // code row stride (a multiple of 64 T)
+  uint32_t ntiles;        // tiles of this evaluation (blocksum entries)
 };
 hipError_t launch_prune_f32(const F32Args &a, uint32_t ppl, bool hbm, uint32_t blocks,
                             hipStream_t s);
 size_t prune_f32_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n,
                            uint32_t ppl);
+// persistent batched form over many instances (one F32Args each)
+hipError_t prepare_prune_f32_batch(int max_lds_bytes);
+hipError_t batch_grid(uint32_t ppl, bool hbm, uint32_t threads, size_t lds, uint32_t *grid);
+hipError_t launch_prune_f32_batch(const F32Args *insts, const uint2 *items, uint32_t nitems,
+                                  uint32_t *queue, uint32_t ppl, bool hbm, uint32_t threads,
+                                  size_t lds, uint32_t grid, hipStream_t s);
 hipError_t prepare_prune_f32(int max_lds_bytes);
 
 // ---- prune64.hip --------------------------------------------------------

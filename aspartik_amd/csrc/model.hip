@@ -133,6 +133,37 @@ __global__ void pbuild_kernel(const Entry *entries, uint32_t n, uint32_t K,
   }
 }
 
+// Batched P(t) build: blockIdx.y selects the instance, whose entries, edge
+// times, rates, model and output come from its descriptor.
+__global__ void pbuild_batch_kernel(const PbuildDesc *desc, uint32_t K, uint32_t S) {
+  const PbuildDesc d = desc[blockIdx.y];
+  const uint32_t SS = S * S;
+  const uint64_t total = (uint64_t)d.n * 2 * K * SS;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t el = (uint32_t)(g % SS);
+    const uint64_t mat = g / SS;
+    const uint32_t c = (uint32_t)(mat % K);
+    const uint64_t es = mat / K;
+    const uint32_t side = (uint32_t)(es & 1);
+    const uint32_t e = (uint32_t)(es >> 1);
+    const uint32_t orig = d.entries[e].meta >> 8;
+    const double t = d.times[2 * orig + side] * d.rates[c];
+    d.out[g] = (float)p_element(d.model, (int)S, t, (int)(el / S), (int)(el % S));
+  }
+}
+
+hipError_t launch_pbuild_batch(const PbuildDesc *desc, uint32_t count, uint32_t max_n,
+                               uint32_t K, uint32_t S, hipStream_t s) {
+  const uint64_t per = (uint64_t)max_n * 2 * K * S * S;
+  uint64_t bx = (per + 255) / 256;
+  if (bx > 1024) bx = 1024;
+  if (bx == 0 || count == 0) return hipSuccess;
+  hipLaunchKernelGGL(pbuild_batch_kernel, dim3((uint32_t)bx, count), dim3(256), 0, s, desc,
+                     K, S);
+  return hipGetLastError();
+}
+
 hipError_t launch_eigen(DevModel *m, hipStream_t s) {
   hipLaunchKernelGGL(eigen_kernel, dim3(1), dim3(64), 0, s, m);
   return hipGetLastError();

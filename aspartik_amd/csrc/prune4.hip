@@ -121,11 +121,13 @@ size_t prune_f32_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n,
 // then fetched one node ahead.  Without it (full evaluations) the rare HBM
 // child (LDS stack overflow) is loaded where it is used, and the prefetch
 // registers are not allocated.
-template <int PPL, bool HBM>
-__global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
+// `A` is the argument record's type: the kernel argument itself, or a record
+// in the constant address space (batched form), whose fields the compiler
+// may reload instead of keeping them in scalar registers.
+template <int PPL, bool HBM, typename A>
+__device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned char *smem) {
   constexpr uint32_t MAT_REGS = 4;        // matrix-chunk dwords per thread
   constexpr uint32_t CODE_REGS = 4 * PPL;  // code-chunk dwords per thread
-  extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t nthr = blockDim.x;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
@@ -134,7 +136,7 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
   const uint32_t W = K * T;
   const uint32_t c = w % K, t = w / K;
   const uint32_t TP = 64u * T * PPL;
-  const uint32_t tile0 = blockIdx.x * TP;
+  const uint32_t tile0 = tile * TP;
   uint32_t p[PPL], pp[PPL];
   bool active[PPL];
 #pragma unroll
@@ -416,10 +418,57 @@ __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
   double bsum = 0.0;
   if (tid == 0)
     for (uint32_t tt = 0; tt < T; tt++) bsum += red[tt];
-  last_block_total(bsum, a.blocksum, a.counter, a.total, flag);
+  last_block_total(bsum, a.blocksum, a.counter, a.total, flag, tile, a.ntiles);
 #undef B3H_FETCH
 #undef B3H_STORE
 #undef B3H_FETCH_HBM
+}
+
+template <int PPL, bool HBM>
+__global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  prune_tile<PPL, HBM>(a, blockIdx.x, smem);
+}
+
+// Persistent batched form: workgroups (at most 512 threads) take
+// (instance, tile) items from a queue (ordered longest first by the host)
+// until it is empty.  The queue
+// counter is reset by the last workgroup to leave, so the kernel can be
+// replayed (also from a captured graph) without a host-side reset.
+template <int PPL, bool HBM>
+__global__ __launch_bounds__(512) void prune_f32_batch_kernel(const F32Args *insts,
+                                                               const uint2 *items,
+                                                               uint32_t nitems,
+                                                               uint32_t *queue) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t item_s;
+  for (;;) {
+    if (threadIdx.x == 0) item_s = atomicAdd(&queue[0], 1u);
+    __syncthreads();
+    const uint32_t item = item_s;
+    __syncthreads();
+    if (item >= nitems) break;
+    const uint2 it = items[item];
+    const uint32_t inst = __builtin_amdgcn_readfirstlane(it.x);
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(it.y);
+#if defined(__HIP_DEVICE_COMPILE__)
+    // the record is read through the constant address space: uniform scalar
+    // loads the compiler can repeat rather than hold in registers
+    typedef const __attribute__((address_space(4))) F32Args ConstArgs;
+    ConstArgs &a = ((ConstArgs *)insts)[inst];
+    prune_tile<PPL, HBM>(a, tile, smem);
+#else
+    prune_tile<PPL, HBM>(insts[inst], tile, smem);
+#endif
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+      __hip_atomic_store(&queue[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&queue[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
 
 hipError_t prepare_prune_f32(int max_lds_bytes) {
@@ -432,6 +481,48 @@ hipError_t prepare_prune_f32(int max_lds_bytes) {
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+hipError_t prepare_prune_f32_batch(int max_lds_bytes) {
+  const void *fns[] = {reinterpret_cast<const void *>(&prune_f32_batch_kernel<1, false>),
+                       reinterpret_cast<const void *>(&prune_f32_batch_kernel<1, true>),
+                       reinterpret_cast<const void *>(&prune_f32_batch_kernel<2, false>)};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       max_lds_bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t batch_grid(uint32_t ppl, bool hbm, uint32_t threads, size_t lds, uint32_t *grid) {
+  const void *f = (ppl == 2 && !hbm)
+                      ? reinterpret_cast<const void *>(&prune_f32_batch_kernel<2, false>)
+                      : hbm ? reinterpret_cast<const void *>(&prune_f32_batch_kernel<1, true>)
+                            : reinterpret_cast<const void *>(&prune_f32_batch_kernel<1, false>);
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, (int)threads, lds);
+  if (e != hipSuccess) return e;
+  int dev = 0, cus = 0;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+    return e;
+  *grid = (uint32_t)((per_cu > 0 ? per_cu : 1) * cus);
+  return hipSuccess;
+}
+
+hipError_t launch_prune_f32_batch(const F32Args *insts, const uint2 *items, uint32_t nitems,
+                                  uint32_t *queue, uint32_t ppl, bool hbm, uint32_t threads,
+                                  size_t lds, uint32_t grid, hipStream_t s) {
+  const dim3 g(grid), b(threads);
+  const uint32_t l = (uint32_t)lds;
+  if (ppl == 2 && !hbm)
+    hipLaunchKernelGGL((prune_f32_batch_kernel<2, false>), g, b, l, s, insts, items, nitems, queue);
+  else if (hbm)
+    hipLaunchKernelGGL((prune_f32_batch_kernel<1, true>), g, b, l, s, insts, items, nitems, queue);
+  else
+    hipLaunchKernelGGL((prune_f32_batch_kernel<1, false>), g, b, l, s, insts, items, nitems, queue);
+  return hipGetLastError();
 }
 
 hipError_t launch_prune_f32(const F32Args &a, uint32_t ppl, bool hbm, uint32_t blocks,

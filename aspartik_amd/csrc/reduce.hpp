@@ -11,25 +11,27 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Every block contributes one partial sum; the block that arrives last sums
-// all partials in a fixed order (64 strided lane sums, then a fixed butterfly)
-// so the result does not depend on block timing.  `flag` is an LDS int.
-// Must be called by all threads of the block.
+// Each of `count` tiles contributes one partial sum at blocksum[index]; the
+// tile that arrives last sums all partials in a fixed order (64 strided lane
+// sums, then a fixed butterfly) so the result does not depend on timing, and
+// resets the counter.  `flag` is an LDS int.  Must be called by all threads
+// of the block.
 __device__ __forceinline__ void last_block_total(double block_value,
                                                  double *blocksum,
                                                  uint32_t *counter,
-                                                 double *total, int *flag) {
+                                                 double *total, int *flag,
+                                                 uint32_t index, uint32_t count) {
   if (threadIdx.x == 0) {
-    blocksum[blockIdx.x] = block_value;
+    blocksum[index] = block_value;
     __threadfence();
     const uint32_t ticket = atomicAdd(counter, 1u);
-    *flag = (ticket == gridDim.x - 1) ? 1 : 0;
+    *flag = (ticket == count - 1) ? 1 : 0;
   }
   __syncthreads();
   if (*flag && threadIdx.x < 64) {
     __threadfence();
     double acc = 0.0;
-    for (uint32_t j = threadIdx.x; j < gridDim.x; j += 64)
+    for (uint32_t j = threadIdx.x; j < count; j += 64)
       acc += __hip_atomic_load(&blocksum[j], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     acc = wave_sum(acc);

@@ -42,7 +42,10 @@ struct Scheduler {
   std::vector<uint8_t> pushed;
   std::vector<uint32_t> base;
 
+  // partial slot of an internal node: two slots per node selected by the
+  // EditBuf active bit, or one slot per node when editbuf is null
   static uint32_t slot_of(uint32_t node, uint32_t L, const uint8_t *editbuf) {
+    if (!editbuf) return node - L;
     return 2u * (node - L) + (editbuf[node - L] & 1u);
   }
 

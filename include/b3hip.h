@@ -179,6 +179,61 @@ b3h_status b3h_gamma_rates(double alpha, uint32_t num_categories,
 /* Bytes of device memory held by the handle. */
 uint64_t b3h_device_bytes(const b3h_calc *calc);
 
+/* ------------------------------------------------------------------------
+ * Batched evaluation: many independent instances (trees, alignments,
+ * models) of uneven size evaluated by one persistent kernel launch, which
+ * takes (instance, pattern tile) work items from a device queue, longest
+ * trees first.  b3 has no batched calculator; this serves many chains or
+ * proposals at once.  Every instance has its own update list, EditBuf slots,
+ * model and weights; all share K and the configuration.
+ * ---------------------------------------------------------------------- */
+typedef struct b3h_batch b3h_batch;
+
+typedef struct b3h_instance {
+  const uint8_t *samples;   /* [num_leaves x num_patterns] codes, as b3h_create */
+  uint32_t num_leaves;
+  uint32_t num_patterns;
+  const uint32_t *weights;  /* [num_patterns] or NULL (all ones)            */
+} b3h_instance;
+
+typedef struct b3h_update {
+  uint32_t n;                      /* list length; 0 = instance not updated  */
+  const uint32_t *internals;       /* [n]   as b3h_propose_times            */
+  const uint32_t *children;        /* [2n]                                   */
+  const double *edge_times;        /* [2n]                                   */
+  const double *category_rates;    /* [K]                                    */
+} b3h_update;
+
+/* flags for b3h_batch_create */
+#define B3H_BATCH_SINGLE_SLOT 1u  /* one partial slot per node: half the
+                                     memory; b3h_batch_reject is unsupported */
+
+b3h_status b3h_batch_create(const b3h_config *cfg, uint32_t count,
+                            const b3h_instance *instances,
+                            uint32_t num_categories, uint32_t flags,
+                            b3h_batch **out);
+b3h_status b3h_batch_destroy(b3h_batch *batch);
+/* Model of one instance (see b3h_set_model). */
+b3h_status b3h_batch_set_model(b3h_batch *batch, uint32_t instance,
+                               const double *q, const double *frequencies,
+                               uint32_t flags);
+/* Evaluate every instance whose update has n > 0, in one launch.  With
+ * B3H_BATCH_GRAPH set in `flags`, the launch sequence is captured into a HIP
+ * graph and replayed while the update shapes repeat. */
+#define B3H_BATCH_GRAPH 1u
+b3h_status b3h_batch_propose_times(b3h_batch *batch, const b3h_update *updates,
+                                   uint32_t flags);
+/* Per-instance totals of the last evaluation (blocks). */
+b3h_status b3h_batch_totals(b3h_batch *batch, double *out);
+b3h_status b3h_batch_likelihoods(b3h_batch *batch, uint32_t instance,
+                                 uint32_t category, double *out);
+b3h_status b3h_batch_accept(b3h_batch *batch);
+b3h_status b3h_batch_reject(b3h_batch *batch);
+b3h_status b3h_batch_profile_enable(b3h_batch *batch, int enable);
+b3h_status b3h_batch_profile_read(b3h_batch *batch, double *kernel_ms,
+                                  uint64_t *launches);
+uint64_t b3h_batch_device_bytes(const b3h_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif
