@@ -10,6 +10,9 @@ internal-node order, parents[root] = TOMBSTONE.
 """
 from __future__ import annotations
 
+import ctypes as C
+import os
+
 import numpy as np
 import scipy.linalg
 
@@ -145,3 +148,65 @@ def expm_transition(q):
 
 def dirichlet(rng, alpha, n):
     return rng.dirichlet(np.full(n, float(alpha)))
+
+
+_SIM = None
+
+
+def _simlib():
+    global _SIM
+    if _SIM is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libb3sim.so")
+        lib = C.CDLL(path)
+        u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+        f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+        lib.b3sim_states.restype = C.c_int
+        lib.b3sim_states.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p, f64p, f64p,
+                                     C.c_uint64, C.c_uint64, C.c_uint32, u8p]
+        _SIM = lib
+    return _SIM
+
+
+def reversible_transitions(q, freqs, times):
+    """P(t) for many t at once from the eigensystem of the symmetrised
+    reversible Q: returns [len(times), S, S]."""
+    q = np.asarray(q, np.float64)
+    pi = np.asarray(freqs, np.float64)
+    d = np.sqrt(pi)
+    Ssym = (d[:, None] * q / d[None, :])
+    Ssym = 0.5 * (Ssym + Ssym.T)
+    w, U = np.linalg.eigh(Ssym)
+    V = U / d[:, None]
+    Vi = U.T * d[None, :]
+    ex = np.exp(np.outer(np.asarray(times, np.float64), w))
+    return np.einsum("ik,tk,kj->tij", V, ex, Vi)
+
+
+def simulate_states_fast(parents, children, heights, q, freqs, rates, num_sites, seed,
+                         threads=None):
+    """Same process as simulate_states (root ~ freqs, per-site category
+    uniform over ``rates``, child ~ row of P(t r_c)) in the multithreaded
+    host simulator; returns uint8 states [L, num_sites]."""
+    parents = np.asarray(parents)
+    N = parents.size
+    L = (N + 1) // 2
+    S = len(freqs)
+    K = len(rates)
+    root = int(np.nonzero(parents == TOMBSTONE)[0][0])
+    lengths = np.zeros(N)
+    nonroot = np.arange(N) != root
+    lengths[nonroot] = heights[parents[nonroot].astype(np.int64)] - heights[nonroot]
+    t = (lengths[:, None] * np.asarray(rates)[None, :]).reshape(-1)
+    P = np.clip(reversible_transitions(q, freqs, t), 0.0, None)
+    cum = np.ascontiguousarray(np.cumsum(P, axis=2).reshape(-1))
+    order = np.ascontiguousarray(sorted(range(L, N), key=lambda n: -heights[n]), np.uint32)
+    out = np.empty((L, num_sites), np.uint8)
+    rc = _simlib().b3sim_states(L, S, K, order,
+                                np.ascontiguousarray(children, np.uint32).reshape(-1), cum,
+                                np.cumsum(np.asarray(freqs, np.float64)), int(num_sites),
+                                int(seed) & (2**64 - 1), int(threads or os.cpu_count() or 1),
+                                out.reshape(-1))
+    if rc != 0:
+        raise ValueError("b3sim_states rejected its arguments")
+    return out

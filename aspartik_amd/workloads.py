@@ -105,3 +105,28 @@ def algorithmic_flops(internals, children, num_leaves, K, P, states=4):
         f += per
     f += 2 * S + 1
     return f * K * P
+
+
+def hky_gamma4_batch(count=256, taxa=(50, 500), patterns=(2_000, 50_000), seed=4, alpha=0.5):
+    """Batched HKY+Gamma4 workload: ``count`` instances, each with its own
+    seeded Yule tree of U{taxa} leaves and U{patterns} sites simulated on it
+    (multithreaded host simulator), kappa ~ LogNormal(1, 0.5), equal base
+    frequencies, alpha 0.5, weights 1.  With at least 50 taxa, duplicate
+    simulated columns are vanishingly rare, so the sites are used as the
+    instance's patterns without compression."""
+    rng = np.random.default_rng(seed)
+    cat = gamma_rates(alpha, 4)
+    out = []
+    for i in range(count):
+        L = int(rng.integers(taxa[0], taxa[1] + 1))
+        P = int(rng.integers(patterns[0], patterns[1] + 1))
+        parents, children, heights = sim.yule_tree(L, rng, 1.0, 0.1)
+        kappa = float(rng.lognormal(1.0, 0.5))
+        freqs = np.full(4, 0.25)
+        model = HKY(freqs, kappa)
+        states = sim.simulate_states_fast(parents, children, heights, model.q_matrix(), freqs,
+                                          cat, P, int(rng.integers(0, 2**62)))
+        out.append(dict(parents=parents, children=children, heights=heights, kappa=kappa,
+                        model=model, codes=sim.dna_codes(states), L=L, P=P))
+    return dict(name=f"hky_gamma4_batch{count}", instances=out, K=4, alpha=alpha, rates=cat,
+                seed=seed)
