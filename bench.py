@@ -115,10 +115,14 @@ def bench_batch(args):
     cpu, err = None, None
     if not args.no_cpu:
         from oracle import oracle as O
-        sample = sorted(range(len(insts)), key=lambda i: insts[i]["L"] * insts[i]["P"])[:6]
+        order = np.random.default_rng(1).permutation(len(insts))
         ncores = len(os.sched_getaffinity(0))
-        rel, nev, t_cpu, work = [], 0, 0.0, 0
-        for i in sample:
+        rel, nev, t_cpu, work, sample = [], 0, 0.0, 0, []
+        for i in order:
+            if t_cpu > args.cpu_seconds or len(sample) >= 24:
+                break
+            i = int(i)
+            sample.append(i)
             x = insts[i]
             ot = O.Tree(x["L"], x["parents"], x["children"], x["heights"])
             om = O.Model.hky(x["kappa"], np.full(4, 0.25))
@@ -141,7 +145,7 @@ def bench_batch(args):
         # instance evaluations per second at the average instance size
         cpu = {"value": (work / t_cpu) / (all_work / len(insts)), "unit": "instance logL evals/s",
                "cores": ncores * K, "kind": "port",
-               "sample": f"{nev} full evaluations of the {len(sample)} smallest instances "
+               "sample": f"{nev} full evaluations of {len(sample)} randomly chosen instances "
                          f"({t_cpu:.1f} s), scaled to the batch's mean instance size by "
                          f"node x pattern count"}
         err = {"max_total_rel": float(max(rel)), "instances_checked": sample}
