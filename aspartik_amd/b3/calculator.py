@@ -37,6 +37,13 @@ class Calculator:
         """FP64 plain pruning: b3's GPU algorithm at FP64 (the baseline)."""
         return cls(_lib.B3H_F64_PLAIN, device)
 
+    @classmethod
+    def HIPCodon(cls, states: int = 61, device: int = 0):
+        """FP16 partials on the MFMA matrix cores (32x32x16 f16 tiles, FP32
+        accumulation) with per-node power-of-two scaling, for 2..64 states
+        (GY94 codon models: 61)."""
+        return cls(_lib.B3H_F16_MFMA, device, states=states)
+
     def with_scale(self, scale_ln: int):
         """calculator/mod.rs:149-155"""
         self.scale_ln = int(scale_ln)

@@ -209,31 +209,59 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
   }
 
   const bool f32 = c->precision == B3H_F32_SCALED;
-  const uint32_t cap = f32 ? c->sched.capacity : 0;
+  const bool codon = c->precision == B3H_F16_MFMA;
+  const bool reorder = f32 || codon;
+  const uint32_t cap = reorder ? c->sched.capacity : 0;
   std::string err;
-  if (!c->sched.build(n, internals, children, c->L, c->editbuf_next.data(),
-                      /*reorder=*/f32, cap, &err))
+  if (!c->sched.build(n, internals, children, c->L, c->editbuf_next.data(), reorder, cap, &err))
     return fail(B3H_ERR_ARG, err);
   c->editbuf.swap(c->editbuf_next);
 
-  // stage: entries | matrices (mode 0) or times + rates (mode 1)
+  // stage: entries | matrices (mode 0) or times + rates (mode 1) | pi
+  // (the MFMA path stages host matrices already in its FP16 A layout)
   const size_t ebytes = (size_t)n * sizeof(Entry);
-  const size_t mbytes = mats ? (size_t)K * 2 * n * SS * 8 : 0;
+  const size_t mbytes = !mats ? 0 : codon ? (size_t)K * 2 * n * 8192 * 2 : (size_t)K * 2 * n * SS * 8;
   const size_t tbytes = times ? (size_t)2 * n * 8 : 0;
   const size_t rbytes = times ? (size_t)K * 8 : 0;
   const size_t off_m = (ebytes + 255) & ~(size_t)255;
   const size_t off_t = off_m + ((mbytes + 255) & ~(size_t)255);
   const size_t off_r = off_t + ((tbytes + 255) & ~(size_t)255);
-  const size_t total = off_r + rbytes;
+  const size_t off_pi = off_r + ((rbytes + 255) & ~(size_t)255);
+  const size_t total = off_pi + 64 * 8;
   if (total > c->stage_bytes) return fail(B3H_ERR_ARG, "staging overflow");
   int s = 0;
   unsigned char *h = next_stage(c, &s);
   std::memcpy(h, c->sched.entries.data(), ebytes);
-  if (mats) std::memcpy(h + off_m, mats, mbytes);
+  if (mats && !codon) std::memcpy(h + off_m, mats, mbytes);
+  if (mats && codon) {
+    // FP16 A layout of the kernel: entry order, [side][category][hi, lo][row]
+    // [slot], slot 16 s + 8 hh + m holding column row(hh, 8 s + m) and
+    // P = hi + 2^-20 lo (prune_codon.hip)
+    _Float16 *dst = reinterpret_cast<_Float16 *>(h + off_m);
+    for (uint32_t e = 0; e < n; e++) {
+      const uint32_t orig = c->sched.entries[e].meta >> 8;
+      for (uint32_t side = 0; side < 2; side++)
+        for (uint32_t cat = 0; cat < K; cat++) {
+          const double *src = mats + ((size_t)cat * 2 * n + 2 * orig + side) * SS;
+          _Float16 *out = dst + ((size_t)(2 * e + side) * K + cat) * 8192;
+          for (uint32_t j = 0; j < 64; j++)
+            for (uint32_t kslot = 0; kslot < 64; kslot++) {
+              const uint32_t q = 8 * (kslot / 16) + kslot % 8, hh = (kslot / 8) & 1u;
+              const uint32_t k = 32u * (q / 16u) + 8u * ((q % 16u) / 4u) + 4u * hh + (q % 4u);
+              const double v = (j < S && k < S) ? src[j * S + k] : 0.0;
+              const _Float16 hi = (_Float16)(float)v;
+              out[j * 64 + kslot] = hi;
+              out[4096 + j * 64 + kslot] = (_Float16)(float)((v - (double)hi) * 0x1p20);
+            }
+        }
+    }
+  }
   if (times) {
     std::memcpy(h + off_t, times, tbytes);
     std::memcpy(h + off_r, rates, rbytes);
   }
+  double *hpi = reinterpret_cast<double *>(h + off_pi);
+  for (uint32_t j = 0; j < 64; j++) hpi[j] = j < S ? freqs[j] : 0.0;
   unsigned char *d = c->d_stage[s];
   HIPCHK(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipEventRecord(c->stage_ev[s], c->stream));
@@ -259,7 +287,42 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     c->ev_used++;
   }
 
-  if (f32) {
+  if (codon) {
+    const _Float16 *amats = reinterpret_cast<const _Float16 *>(d + off_m);
+    if (mode == 1) {
+      HIPCHK(launch_pbuild_codon(d_entries, n, K, d_times, d_rates, c->d_model,
+                                 static_cast<_Float16 *>(c->d_pmats), c->stream));
+      amats = static_cast<const _Float16 *>(c->d_pmats);
+    }
+    CodonArgs a{};
+    a.entries = d_entries;
+    a.amats = amats;
+    a.codes = c->d_codes;
+    a.partials = static_cast<_Float16 *>(c->d_partials);
+    a.exps = c->d_exps;
+    a.site = c->d_site;
+    a.pi = reinterpret_cast<const double *>(d + off_pi);
+    a.cstride = c->cstride;
+    a.n = n;
+    a.P = c->P;
+    a.Ppad = c->cstride;
+    a.K = K;
+    a.S = S;
+    if (e0) HIPCHK(hipEventRecord(e0, c->stream));
+    HIPCHK(launch_prune_codon(a, c->stream));
+    if (e1) HIPCHK(hipEventRecord(e1, c->stream));
+    MixArgs m{};
+    m.site = c->d_site;
+    m.mixed = c->d_mixed;
+    m.weights = c->d_weights;
+    m.blocksum = c->d_mix_blocksum;
+    m.counter = c->d_counter;
+    m.total = c->d_total;
+    m.ln_inv_k = std::log(1.0 / (double)K);
+    m.P = c->P;
+    m.K = K;
+    HIPCHK(launch_mix(m, c->stream));
+  } else if (f32) {
     HIPCHK(launch_pbuild<float>(d_entries, n, K, S, mode, d_mats, d_times,
                                 d_rates, c->d_model,
                                 static_cast<float *>(c->d_pmats), c->stream));
@@ -379,10 +442,15 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   if (num_categories == 0 || num_categories > 16)
     return fail(B3H_ERR_ARG, "num_categories must be 1..16");
   if (cfg->scale_ln == 0) return fail(B3H_ERR_ARG, "scale_ln must be nonzero");
-  if (cfg->precision != B3H_F32_SCALED && cfg->precision != B3H_F64_PLAIN)
-    return fail(B3H_ERR_UNSUPPORTED, "precision not supported by this entry point");
-  if (cfg->states != 4)
-    return fail(B3H_ERR_UNSUPPORTED, "this entry point supports 4 states");
+  if (cfg->precision != B3H_F32_SCALED && cfg->precision != B3H_F64_PLAIN &&
+      cfg->precision != B3H_F16_MFMA)
+    return fail(B3H_ERR_UNSUPPORTED, "unknown precision");
+  if (cfg->precision == B3H_F16_MFMA) {
+    if (cfg->states < 2 || cfg->states > 64)
+      return fail(B3H_ERR_UNSUPPORTED, "the MFMA path supports 2..64 states");
+  } else if (cfg->states != 4) {
+    return fail(B3H_ERR_UNSUPPORTED, "the FP32 and FP64 paths support 4 states");
+  }
   if ((uint64_t)num_leaves * 2 >= (1u << 24))
     return fail(B3H_ERR_ARG, "too many leaves");
 
@@ -398,7 +466,7 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   c->I = num_leaves - 1;
   c->P = num_patterns;
   c->K = num_categories;
-  c->S = 4;
+  c->S = cfg->states;
   c->scale_ln = cfg->scale_ln;
   c->editbuf.assign(c->I, 0);
   c->has_result = false;
@@ -421,45 +489,61 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   const uint32_t K = c->K, P = c->P, S = c->S;
   const size_t slots = (size_t)2 * c->I;
   const bool f32 = c->precision == B3H_F32_SCALED;
-
-  // Tile choice: T wavefront rows per category (64*K*T <= 1024 threads) and
-  // PPL patterns per lane.  Two patterns per lane once the wavefront count
-  // (K*P/64) exceeds what the chip holds at once (~16 per CU x 256 CUs).
-  uint32_t PPL = cfg->lane_patterns;
-  if (PPL == 0) PPL = ((uint64_t)K * P / 64 > 4096) ? 2 : 1;
-  uint32_t T = 1;
-  if (cfg->tile_patterns) T = std::max<uint32_t>(1u, cfg->tile_patterns / 64u);
-  else T = std::max<uint32_t>(1u, 16u / K);  // 1024-thread workgroups (measured best)
-  while (64u * K * T > 1024u && T > 1) T--;
-  if (PPL > 2) {
-    fail(B3H_ERR_ARG, "lane_patterns must be 0, 1 or 2");
-    return bail(B3H_ERR_ARG);
-  }
-  c->T = T;
-  c->PPL = PPL;  // upper bound; partial updates launch with one pattern per lane
-  const uint32_t TP = 64 * T * PPL;
-  c->blocks = (P + 64 * T - 1) / (64 * T);  // the most blocks any launch uses
-
-  // LDS stack capacity for this tile shape and the longest update list
-  uint32_t cap = kMaxSlots;
-  while (cap > 1 && prune_f32_lds_bytes(K, T, cap, c->I, PPL) > kMaxLds) cap--;
-  if (prune_f32_lds_bytes(K, T, cap, c->I, PPL) > kMaxLds) {
-    fail(B3H_ERR_UNSUPPORTED, "tree too large for the LDS-staged update list");
-    return bail(B3H_ERR_UNSUPPORTED);
-  }
-  c->sched.capacity = cap;
-
+  const bool codon = c->precision == B3H_F16_MFMA;
   b3h_status st;
-  c->cstride = (P + TP - 1) / TP * TP;  // whole tiles, no over-read
+
+  if (codon) {
+    // MFMA path: 32-pattern tiles, four per 256-thread workgroup, patterns
+    // padded to a multiple of 128; FP16 partials of 64 states; no LDS stack
+    // (children not in registers are read back from HBM)
+    c->T = 4;
+    c->PPL = 1;
+    c->cstride = (P + 127) / 128 * 128;
+    if (prune_codon_lds_bytes(c->I) > kMaxLds) {
+      fail(B3H_ERR_UNSUPPORTED, "tree too large for the LDS-staged update list");
+      return bail(B3H_ERR_UNSUPPORTED);
+    }
+    c->sched.capacity = 0;
+  } else {
+    // Tile choice: T wavefront rows per category (64*K*T <= 1024 threads) and
+    // PPL patterns per lane.  Two patterns per lane once the wavefront count
+    // (K*P/64) exceeds what the chip holds at once (~16 per CU x 256 CUs).
+    uint32_t PPL = cfg->lane_patterns;
+    if (PPL == 0) PPL = ((uint64_t)K * P / 64 > 4096) ? 2 : 1;
+    uint32_t T = 1;
+    if (cfg->tile_patterns) T = std::max<uint32_t>(1u, cfg->tile_patterns / 64u);
+    else T = std::max<uint32_t>(1u, 16u / K);  // 1024-thread workgroups (measured best)
+    while (64u * K * T > 1024u && T > 1) T--;
+    if (PPL > 2) {
+      fail(B3H_ERR_ARG, "lane_patterns must be 0, 1 or 2");
+      return bail(B3H_ERR_ARG);
+    }
+    c->T = T;
+    c->PPL = PPL;  // upper bound; partial updates launch with one pattern per lane
+    const uint32_t TP = 64 * T * PPL;
+    c->blocks = (P + 64 * T - 1) / (64 * T);  // the most blocks any launch uses
+    // LDS stack capacity for this tile shape and the longest update list
+    uint32_t cap = kMaxSlots;
+    while (cap > 1 && prune_f32_lds_bytes(K, T, cap, c->I, PPL) > kMaxLds) cap--;
+    if (prune_f32_lds_bytes(K, T, cap, c->I, PPL) > kMaxLds) {
+      fail(B3H_ERR_UNSUPPORTED, "tree too large for the LDS-staged update list");
+      return bail(B3H_ERR_UNSUPPORTED);
+    }
+    c->sched.capacity = cap;
+    c->cstride = (P + TP - 1) / TP * TP;  // whole tiles, no over-read
+  }
+
   std::vector<uint8_t> codes((size_t)c->L * c->cstride, (uint8_t)S);
   for (size_t l = 0; l < c->L; l++)
     for (size_t i = 0; i < P; i++)
       codes[l * c->cstride + i] = remap_code(samples[l * P + i], S);
   if ((st = dev_alloc(c, (void **)&c->d_codes, codes.size()))) return bail(st);
-  const size_t elem = f32 ? 16 : 32;
-  if ((st = dev_alloc(c, &c->d_partials, slots * K * P * elem))) return bail(st);
-  if (f32) {
-    if ((st = dev_alloc(c, (void **)&c->d_exps, slots * K * P * 2))) return bail(st);
+  // partial store, one "pattern unit" per (slot, category, pattern)
+  const size_t punits = codon ? slots * K * c->cstride : slots * K * P;
+  const size_t elem = codon ? 128 : f32 ? 16 : 32;
+  if ((st = dev_alloc(c, &c->d_partials, punits * elem))) return bail(st);
+  if (f32 || codon) {
+    if ((st = dev_alloc(c, (void **)&c->d_exps, punits * 2))) return bail(st);
   } else {
     if ((st = dev_alloc(c, (void **)&c->d_scales, slots * K * P))) return bail(st);
     if ((st = dev_alloc(c, (void **)&c->d_scale_sums, (size_t)K * P * 4))) return bail(st);
@@ -468,17 +552,22 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   if ((st = dev_alloc(c, (void **)&c->d_site, (size_t)K * P * 8))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_mixed, (size_t)P * 8))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_weights, (size_t)P * 4))) return bail(st);
-  if ((st = dev_alloc(c, (void **)&c->d_blocksum, (size_t)c->blocks * 8))) return bail(st);
+  if ((st = dev_alloc(c, (void **)&c->d_blocksum, (size_t)std::max<uint32_t>(c->blocks, 1u) * 8)))
+    return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_mix_blocksum, (size_t)mix_blocks(P) * 8))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_counter, 64))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_total, 64))) return bail(st);
-  if ((st = dev_alloc(c, &c->d_pmats, (size_t)c->I * 2 * K * S * S * (f32 ? 4 : 8)))) return bail(st);
+  const size_t mat_bytes = codon ? 8192 * 2 : (size_t)S * S * (f32 ? 4 : 8);
+  if ((st = dev_alloc(c, &c->d_pmats, (size_t)c->I * 2 * K * mat_bytes))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_model, sizeof(DevModel)))) return bail(st);
 
-  // staging: entries + f64 matrices (largest mode) + times + rates
+  // staging: entries + host matrices (the larger of the f64 input and the
+  // FP16 A layout) + times + rates + stationary frequencies
+  const size_t host_mat = std::max<size_t>((size_t)S * S * 8, codon ? 8192 * 2 : 0);
   c->stage_bytes = ((size_t)c->I * sizeof(Entry) + 255) / 256 * 256 +
-                   ((size_t)K * 2 * c->I * S * S * 8 + 255) / 256 * 256 +
-                   ((size_t)2 * c->I * 8 + 255) / 256 * 256 + (size_t)K * 8 + 256;
+                   ((size_t)K * 2 * c->I * host_mat + 255) / 256 * 256 +
+                   ((size_t)2 * c->I * 8 + 255) / 256 * 256 + ((size_t)K * 8 + 255) / 256 * 256 +
+                   64 * 8 + 256;
   for (int s = 0; s < kStages; s++) {
     if (hipHostMalloc((void **)&c->h_stage[s], c->stage_bytes, hipHostMallocDefault) != hipSuccess) {
       fail(B3H_ERR_OOM, "hipHostMalloc staging failed");
@@ -495,9 +584,10 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   bool ok = hipMemcpy(c->d_codes, codes.data(), codes.size(), hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(c->d_weights, ones.data(), (size_t)P * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemset(c->d_counter, 0, 64) == hipSuccess &&
-            hipMemset(c->d_partials, 0, slots * K * P * elem) == hipSuccess;
-  if (ok && f32) ok = hipMemset(c->d_exps, 0, slots * K * P * 2) == hipSuccess;
-  if (ok && !f32)
+            hipMemset(c->d_model, 0, sizeof(DevModel)) == hipSuccess &&
+            hipMemset(c->d_partials, 0, punits * elem) == hipSuccess;
+  if (ok && (f32 || codon)) ok = hipMemset(c->d_exps, 0, punits * 2) == hipSuccess;
+  if (ok && !f32 && !codon)
     ok = hipMemset(c->d_scales, 0, slots * K * P) == hipSuccess &&
          hipMemset(c->d_scale_sums, 0, (size_t)K * P * 4) == hipSuccess &&
          hipMemset(c->d_scale_sums_bak, 0, (size_t)K * P * 4) == hipSuccess;
@@ -506,7 +596,8 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
     fail(B3H_ERR_HIP, "initial device upload failed");
     return bail(B3H_ERR_HIP);
   }
-  if (f32 && prepare_prune_f32((int)kMaxLds) != hipSuccess) {
+  if ((f32 && prepare_prune_f32((int)kMaxLds) != hipSuccess) ||
+      (codon && prepare_prune_codon((int)kMaxLds) != hipSuccess)) {
     fail(B3H_ERR_HIP, "cannot raise the kernel's LDS limit");
     return bail(B3H_ERR_HIP);
   }

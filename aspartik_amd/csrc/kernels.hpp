@@ -59,6 +59,26 @@ hipError_t launch_prune_f32_batch(const F32Args *insts, const uint2 *items, uint
                                   size_t lds, uint32_t grid, hipStream_t s);
 hipError_t prepare_prune_f32(int max_lds_bytes);
 
+// ---- prune_codon.hip ----------------------------------------------------
+// S <= 64 states on MFMA: FP16 partials [slot][K][Ppad/32][64 lanes][32],
+// FP16 matrices in the kernel's A layout [n][2][K][64][64].
+struct CodonArgs {
+  const Entry *entries;
+  const _Float16 *amats;   // [entry][side][K][hi, lo][64][64], permuted columns
+  const uint8_t *codes;    // [L][cstride] state indices, >= S unknown
+  _Float16 *partials;      // [slot][K][Ppad / 32][4][64][8]
+  int16_t *exps;           // [slot][K][Ppad]
+  double *site;            // [K][P]
+  const double *pi;        // [64], zero beyond S
+  uint32_t cstride, n, P, Ppad, K, S;
+};
+size_t prune_codon_lds_bytes(uint32_t n);
+hipError_t prepare_prune_codon(int max_lds_bytes);
+hipError_t launch_prune_codon(const CodonArgs &a, hipStream_t s);
+hipError_t launch_pbuild_codon(const Entry *entries, uint32_t n, uint32_t K, const double *times,
+                               const double *rates, const DevModel *model, _Float16 *out,
+                               hipStream_t s);
+
 // ---- prune64.hip --------------------------------------------------------
 // FP64 plain baseline: one lane per (pattern, category), b3's e^scale_ln
 // flag scaling, children read from HBM.
