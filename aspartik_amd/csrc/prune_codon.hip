@@ -37,10 +37,11 @@
 // update list in the depth-first order of the host scheduler.  The child
 // evaluated just before its parent stays in registers; every other internal
 // child is read back from HBM (each node's partial is stored anyway), and
-// that read, like the tip codes, is issued ahead of its node.  The four
-// wavefronts of a workgroup share the current node's matrices in LDS; each
-// side's buffer is refilled with the next node's matrices (loaded into
-// registers half a node earlier) once all four have used it.
+// that read, like the tip codes, is issued a node ahead, straight into LDS.
+// The four wavefronts of a workgroup share the current node's matrices in
+// LDS; the next node's are loaded into registers a node ahead and written
+// between the two barriers of a node.  Each wavefront runs its two
+// children's product chains interleaved, one 32-row block at a time.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -52,13 +53,19 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
-constexpr uint32_t kRowH = 72;           // halves per LDS matrix row (64 + 8 pad)
-constexpr uint32_t kMatH = 64 * kRowH;   // halves per LDS matrix
+constexpr uint32_t kMatH = 64 * 64;      // halves per LDS matrix
 constexpr float kLoScale = 0x1p-20f;     // P = hi + kLoScale * lo
 constexpr int kXTop = 15;                // stored partials: pattern max in [2^14, 2^15)
 constexpr uint32_t kWaves = 4;           // tiles (wavefronts) per workgroup
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Workgroup barrier for the LDS matrix buffers: waits for this wavefront's
+// LDS operations only.  (__syncthreads also waits for every outstanding
+// global load, including the LDS-direct prefetches of later nodes.)
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
 
 // state held in register i by a lane of half h
 __device__ __forceinline__ uint32_t row_of(uint32_t h, uint32_t i) {
@@ -69,61 +76,50 @@ struct Tile {
   h8 v[4];  // 32 halves in register order
 };
 
-// row_of(h, i) = row_of(0, i) + 4 h: lane-dependent tests are written against
-// the compile-time row_of(0, i) so that no per-lane table stays live.
-//
-// B operand of step s for a tip: one-hot over the lane's states of that
-// step; d = code - 4 h (as an int, so codes below 4 h match nothing).
-__device__ __forceinline__ h8 one_hot_step(int d, uint32_t s) {
-  h8 b;
-#pragma unroll
-  for (uint32_t m = 0; m < 8; m++)
-    b[m] = (int)row_of(0, 8 * s + m) == d ? (_Float16)1.0f : (_Float16)0.0f;
-  return b;
-}
+// LDS matrices are row-major with the 16-byte chunks of row r stored at
+// chunk ^ (r & 7), so the eight lanes of a b128 read phase hit distinct banks.
 
-// One 32-row block of a child's product, (P x)[32 rb + ..] for the lane's
-// pattern, from the child's two matrix parts in LDS (hi at A, lo at
-// A + kMatH); bstep(s) gives the B operand of step s.
-template <typename BStep>
-__device__ __forceinline__ f16v row_block(const _Float16 *A, uint32_t rb, uint32_t col, uint32_t h,
-                                          BStep bstep) {
-  const _Float16 *hi = A + (32 * rb + col) * kRowH + 8 * h;
-  const _Float16 *lo = hi + kMatH;
-  f16v acc = {};
-#pragma unroll
-  for (uint32_t s = 0; s < 4; s++)
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(*reinterpret_cast<const h8 *>(lo + 16 * s),
-                                                 bstep(s), acc, 0, 0, 0);
-  acc *= kLoScale;
-#pragma unroll
-  for (uint32_t s = 0; s < 4; s++)
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(*reinterpret_cast<const h8 *>(hi + 16 * s),
-                                                 bstep(s), acc, 0, 0, 0);
-  return acc;
+// max / sum with the lane holding the same pattern in the other half
+__device__ __forceinline__ float pair_max(float v) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false,
+                                                  false);
+  return fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
 }
-
-// Sources of one entry as the node loop consumes them: tip codes, and the
-// first child read from HBM (tile + exponent), loaded ahead of their node.
-struct Pending {
-  uint32_t code[2];
-  Tile x;
-  int e;
-};
+__device__ __forceinline__ double pair_sum(double v) {
+  const uint64_t b = __double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)b, (uint32_t)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(b >> 32), (uint32_t)(b >> 32),
+                                                   false, false);
+  return __longlong_as_double((int64_t)(((uint64_t)hi[0] << 32) | lo[0])) +
+         __longlong_as_double((int64_t)(((uint64_t)hi[1] << 32) | lo[1]));
+}
 
 }  // namespace
 
-// LDS: entries | stationary frequencies (64 doubles) | matrices [side][part]
-size_t prune_codon_lds_bytes(uint32_t n) {
-  return ((size_t)n * 16 + 15) / 16 * 16 + 64 * 8 + 4 * (size_t)kMatH * 2;
-}
+// All LDS is static (77 KiB: matrices of both sides, per-wavefront child
+// tiles, codes and exponents, the one-hot table, frequencies); distinct
+// arrays let the compiler see that the LDS-direct prefetches touch none of
+// the others.  Two workgroups fit a CU.
+size_t prune_codon_lds_bytes(uint32_t) { return 0; }
 
 // grid (Ppad / 128, K): blockIdx.y is the rate category; 4 wavefronts per
 // workgroup, each owning one 32-pattern tile (the pattern count is padded to
 // a multiple of 128, so every wavefront's tile exists and every access of
 // the padded codes, partials and exponents is in bounds).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void prune_codon_kernel(CodonArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
+//
+// Per node i the global memory operations are issued in a fixed order on
+// every path, so that each wait covers only what it needs:
+//   node start   12 LDS-direct loads: entry i+1's tip codes (2) and the tiles
+//                (4 + 4) and exponents (1 + 1) of its HBM children; absent
+//                sources read a shared dummy tile so that the count never
+//                varies
+//   barrier      (after the products) store entry i+1's matrices (loaded
+//                during node i-1), copy entry i+1's prefetched sources from
+//                LDS to registers, load entry i+2's matrices
+//   node end     the node's partial (4 stores) and exponent (1 store), then
+//                a barrier that publishes entry i+1's matrices
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u, h = lane / 32u, col = lane % 32u;
   const uint32_t t = uni(tid >> 6);
@@ -133,144 +129,174 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
   const uint32_t p = tile * 32u + col;
   const size_t tiles_all = a.Ppad / 32u;
   // [s][lane][8] tile of (slot, category), and the lane's exponent
-  auto tile_ptr = [&](uint32_t slot) -> _Float16 * {
-    return a.partials + (((size_t)slot * K + c) * tiles_all + tile) * 2048u + (size_t)lane * 8u;
-  };
-  auto exp_idx = [&](uint32_t slot) -> size_t { return ((size_t)slot * K + c) * a.Ppad + p; };
+  _Float16 *const tile_base = a.partials + ((size_t)c * tiles_all + tile) * 2048u + lane * 8u;
+  int16_t *const exp_base = a.exps + (size_t)c * a.Ppad + p;
+  const size_t tile_stride = (size_t)K * tiles_all * 2048u, exp_stride = (size_t)K * a.Ppad;
+  auto tile_ptr = [&](uint32_t slot) -> _Float16 * { return tile_base + slot * tile_stride; };
+  auto exp_ptr = [&](uint32_t slot) -> int16_t * { return exp_base + slot * exp_stride; };
 
-  uint4 *ent_s = reinterpret_cast<uint4 *>(smem);
-  double *pi_s = reinterpret_cast<double *>(smem + ((size_t)n * 16 + 15) / 16 * 16);
-  _Float16 *amat = reinterpret_cast<_Float16 *>(pi_s + 64);
-
-  const uint4 *g_ent = reinterpret_cast<const uint4 *>(a.entries);
-  for (uint32_t i = tid; i < n; i += blockDim.x) ent_s[i] = g_ent[i];
+  __shared__ double pi_s[64];
+  __shared__ __align__(16) _Float16 amat0[2 * kMatH];  // side 0: hi, lo
+  __shared__ __align__(16) _Float16 amat1[2 * kMatH];  // side 1
+  // LDS-direct prefetch targets, per wavefront and side: child tile
+  // [s][lane][8]; codes and exponents land one dword per lane (only the first
+  // 32 bytes, resp. 64 bytes, are the wavefront's patterns)
+  __shared__ __align__(16) _Float16 tbuf[kWaves][2][2048];
+  __shared__ __align__(16) uint32_t cbuf[kWaves][2][64];
+  __shared__ __align__(16) uint32_t ebuf[kWaves][2][64];
+  // B operands of tips: hot[code][h] is the lane's 32 states (register
+  // order) one-hot at `code`; code 64 (any code >= 64) is all zeros.  An
+  // unknown code's product is replaced by the all-ones column anyway.
+  __shared__ __align__(16) _Float16 hot[65][2][32];
   if (tid < 64) pi_s[tid] = a.pi[tid];
+  for (uint32_t e = tid; e < 65u * 64u; e += 256u) {
+    const uint32_t code = e / 64u, hh = (e / 32u) & 1u, r = e % 32u;
+    hot[code][hh][r] = row_of(hh, r) == code ? (_Float16)1.0f : (_Float16)0.0f;
+  }
 
-  // matrices of entry e, side, category c: 2 parts x 64 x 64 halves = 1024
-  // units of 16 B, four per thread of the 256-thread workgroup.  The two
-  // sides are staged separately: one side's LDS buffer is refilled with the
-  // next node's matrices as soon as every wavefront has used it.
-  u4 m0, m1, m2, m3;
-  auto fetch_side = [&](uint32_t e, uint32_t side) {
-    const u4 *src =
-        reinterpret_cast<const u4 *>(a.amats + ((size_t)(2 * e + side) * K + c) * 8192u) + tid;
-    m0 = src[0];
-    m1 = src[256];
-    m2 = src[512];
-    m3 = src[768];
-  };
-  // unit w = tid + 256 q: part w / 512, row (w / 8) % 64, 16-byte chunk w % 8
-  const uint32_t srow = (tid >> 3) & 31u, schunk = tid & 7u;
-  auto store_side = [&](uint32_t side) {
-    _Float16 *d = amat + side * 2u * kMatH + srow * kRowH + schunk * 8u;
-    *reinterpret_cast<u4 *>(d) = m0;
-    *reinterpret_cast<u4 *>(d + 32u * kRowH) = m1;
-    *reinterpret_cast<u4 *>(d + kMatH) = m2;
-    *reinterpret_cast<u4 *>(d + kMatH + 32u * kRowH) = m3;
-  };
-  auto load_tile = [&](uint32_t slot, Tile &x) {
-    const h8 *g = reinterpret_cast<const h8 *>(tile_ptr(slot));
-#pragma unroll
-    for (uint32_t s = 0; s < 4; s++) x.v[s] = g[s * 64u];
-  };
-  // issue the loads of entry i's tip codes and first HBM child
-  Pending pend;
-  auto prefetch = [&](uint32_t i) {
-    const uint4 e4 = ent_s[i];
-    const uint32_t srcs[2] = {uni(e4.y), uni(e4.z)};
-    bool got = false;
+  // matrices of entry e, category c: 2 sides x 2 parts x 64 x 64 halves =
+  // 2048 units of 16 B, eight per thread of the 256-thread workgroup, loaded
+  // into registers one node ahead and stored once every wavefront is done
+  // with the current node's.
+  u4 m[8];
+  auto fetch_mats = [&](uint32_t e) {
 #pragma unroll
     for (uint32_t side = 0; side < 2; side++) {
-      const uint32_t tag = srcs[side] & SRC_TAG, idx = srcs[side] & SRC_IDX;
-      pend.code[side] = tag == SRC_TIP ? a.codes[(size_t)idx * a.cstride + p] : 0u;
-      if (tag == SRC_HBM && !got) {
-        load_tile(idx, pend.x);
-        pend.e = a.exps[exp_idx(idx)];
-        got = true;
-      }
+      const u4 *src =
+          reinterpret_cast<const u4 *>(a.amats + ((size_t)(2 * e + side) * K + c) * 8192u) + tid;
+#pragma unroll
+      for (uint32_t q = 0; q < 4; q++) m[4 * side + q] = src[256u * q];
     }
   };
-  auto has_hbm = [&](uint32_t i) {
-    const uint4 e4 = ent_s[i];
-    return ((uni(e4.y) & SRC_TAG) == SRC_HBM) || ((uni(e4.z) & SRC_TAG) == SRC_HBM);
+  // unit w = tid + 256 q: part w / 512, row (w / 8) % 64, 16-byte chunk w % 8
+  // (rows srow and srow + 32 share the swizzle)
+  const uint32_t srow = (tid >> 3) & 31u;
+  const uint32_t soff = srow * 64u + 8u * ((tid & 7u) ^ (srow & 7u));
+  auto store_mats = [&]() {
+#pragma unroll
+    for (uint32_t side = 0; side < 2; side++) {
+      _Float16 *d = (side == 0 ? amat0 : amat1) + soff;
+      *reinterpret_cast<u4 *>(d) = m[4 * side + 0];
+      *reinterpret_cast<u4 *>(d + 32u * 64u) = m[4 * side + 1];
+      *reinterpret_cast<u4 *>(d + kMatH) = m[4 * side + 2];
+      *reinterpret_cast<u4 *>(d + kMatH + 32u * 64u) = m[4 * side + 3];
+    }
   };
 
-  __syncthreads();  // entries
-  fetch_side(0, 0);
-  store_side(0);
-  fetch_side(0, 1);
-  store_side(1);
-  if (n > 1) fetch_side(1, 0);
-  prefetch(0);
-  __syncthreads();  // matrices of entry 0
-
-  Tile x;
+  // Sources of entry i into LDS.  Tip rows are read 4 bytes per lane at
+  // pattern offset 4 (lane % 8); exponents 2 patterns per lane at offset
+  // 2 (lane % 16).  A side that is not a tip reads tip row 0, and a side that
+  // is not an HBM child reads the tile and exponents of (slot 0, category 0,
+  // tile 0), the same few lines for every wavefront.
+  const _Float16 *const dummy_tile = a.partials + lane * 8u;
+  const int16_t *const dummy_exp = a.exps + 2u * (lane % 16u);
+  auto prefetch = [&](uint32_t i) {
+    const uint4 e4 = ents[i];
+    const uint32_t src[2] = {e4.y, e4.z};
+    const uint8_t *crow = a.codes + (size_t)tile * 32u + 4u * (lane % 8u);
 #pragma unroll
-  for (uint32_t s = 0; s < 4; s++) x.v[s] = h8{};
+    for (uint32_t side = 0; side < 2; side++) {
+      const uint32_t tag = src[side] & SRC_TAG, idx = src[side] & SRC_IDX;
+      __builtin_amdgcn_global_load_lds(crow + (size_t)(tag == SRC_TIP ? idx : 0u) * a.cstride,
+                                       &cbuf[t][side][0], 4, 0, 0);
+      const _Float16 *g = tag == SRC_HBM ? tile_ptr(idx) : dummy_tile;
+      const int16_t *ge = tag == SRC_HBM ? exp_ptr(idx) - col + 2u * (lane % 16u) : dummy_exp;
+#pragma unroll
+      for (uint32_t s = 0; s < 4; s++)
+        __builtin_amdgcn_global_load_lds(g + s * 512u, &tbuf[t][side][s * 512u], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(ge, &ebuf[t][side][0], 4, 0, 0);
+    }
+  };
+  // the prefetched sources of the next entry, in registers
+  Tile y[2];
+  uint32_t code[2] = {0u, 0u};
+  int ey[2] = {0, 0};
+  auto take = [&]() {
+#pragma unroll
+    for (uint32_t side = 0; side < 2; side++) {
+#pragma unroll
+      for (uint32_t s = 0; s < 4; s++)
+        y[side].v[s] = *reinterpret_cast<const h8 *>(&tbuf[t][side][s * 512u + lane * 8u]);
+      code[side] = reinterpret_cast<const uint8_t *>(&cbuf[t][side][0])[col];
+      ey[side] = reinterpret_cast<const int16_t *>(&ebuf[t][side][0])[col];
+    }
+  };
+
+  fetch_mats(0);
+  __syncthreads();  // frequencies, one-hot table
+  store_mats();
+  prefetch(0);
+  if (n > 1) fetch_mats(1);
+  __syncthreads();  // matrices of entry 0, prefetched sources of entry 0
+  take();
+
   int E = 0;
   for (uint32_t i = 0; i < n; i++) {
-    const uint4 e4 = ent_s[i];
-    const uint32_t out = uni(e4.x), srcs[2] = {uni(e4.y), uni(e4.z)};
+    const uint4 e4 = ents[i];
+    const uint32_t out = e4.x, srcs[2] = {e4.y, e4.z};
     const bool more = i + 1 < n;
-    const uint32_t code[2] = {pend.code[0], pend.code[1]};
-    // without an HBM child here the next entry's loads can go out at once;
-    // otherwise right after the HBM child's product has consumed them
-    bool issued = !has_hbm(i);
-    if (issued && more) prefetch(i + 1);
+    if (more) prefetch(i + 1);
 
-    // side 0's product into acc, side 1's multiplied into it block by block
-    f16v acc[2];
+    // B operands and exponents of both children: a tip's one-hot column, or
+    // y[side], which holds the HBM child or (put there at the end of the
+    // previous node) the previous result
+    Tile b[2];
     int ce[2];
 #pragma unroll
     for (uint32_t side = 0; side < 2; side++) {
-      const uint32_t tag = srcs[side] & SRC_TAG, idx = srcs[side] & SRC_IDX;
-      const _Float16 *A = amat + side * 2u * kMatH;
-      // one row block at a time (the barrier keeps the scheduler from
-      // interleaving the two chains, which would double the live registers)
-      auto combine = [&](uint32_t rb, const f16v &r) {
-        if (side == 0) acc[rb] = r;
-        else acc[rb] *= r;
-        __builtin_amdgcn_sched_barrier(0);
-      };
-      if (tag == SRC_TIP) {
+      if ((srcs[side] & SRC_TAG) == SRC_TIP) {
         const uint32_t cd = code[side];
-        const int d = (int)cd - 4 * (int)h, lim = (int)S - 4 * (int)h;
+        const _Float16 *hb = &hot[cd < 64u ? cd : 64u][h][0];
 #pragma unroll
-        for (uint32_t rb = 0; rb < 2; rb++) {
-          f16v r = row_block(A, rb, col, h, [&](uint32_t st) { return one_hot_step(d, st); });
-          if (cd >= S) {  // unknown: the all-ones column
-#pragma unroll
-            for (uint32_t q = 0; q < 16; q++) r[q] = (int)row_of(0, 16 * rb + q) < lim ? 1.0f : 0.0f;
-          }
-          combine(rb, r);
-        }
+        for (uint32_t s = 0; s < 4; s++) b[side].v[s] = *reinterpret_cast<const h8 *>(hb + 8 * s);
         ce[side] = 0;
-      } else if (tag == SRC_PREV) {
-#pragma unroll
-        for (uint32_t rb = 0; rb < 2; rb++)
-          combine(rb, row_block(A, rb, col, h, [&](uint32_t st) { return x.v[st]; }));
-        ce[side] = E;
-      } else if (!issued) {
-#pragma unroll
-        for (uint32_t rb = 0; rb < 2; rb++)
-          combine(rb, row_block(A, rb, col, h, [&](uint32_t st) { return pend.x.v[st]; }));
-        ce[side] = pend.e;
-        issued = true;
-        if (more) prefetch(i + 1);
-      } else {  // a second child from HBM (partial updates only)
-        Tile y;
-        load_tile(idx, y);
-#pragma unroll
-        for (uint32_t rb = 0; rb < 2; rb++)
-          combine(rb, row_block(A, rb, col, h, [&](uint32_t st) { return y.v[st]; }));
-        ce[side] = a.exps[exp_idx(idx)];
+      } else {
+        b[side] = y[side];
+        ce[side] = ey[side];
       }
-      if (more) {
-        __syncthreads();  // every wavefront is done with this side's matrices
-        store_side(side);  // the next entry's
-        if (side == 0) fetch_side(i + 1, 1);
-        else if (i + 2 < n) fetch_side(i + 2, 0);
+    }
+    // per row block, the two children's chains interleaved step by step:
+    // lo parts, the 2^-20 scale, then hi parts accumulated on top; unknown
+    // tips take the all-ones column (row_of(h, i) = row_of(0, i) + 4 h)
+    f16v acc[2];
+    const uint32_t sw = col & 7u;
+    const int lim = (int)S - 4 * (int)h;
+#pragma unroll
+    for (uint32_t rb = 0; rb < 2; rb++) {
+      f16v r[2] = {};
+#pragma unroll
+      for (uint32_t part = 0; part < 2; part++) {
+        const uint32_t po = (part == 0 ? kMatH : 0u) + (32 * rb + col) * 64;  // lo first
+#pragma unroll
+        for (uint32_t s = 0; s < 4; s++)
+#pragma unroll
+          for (uint32_t side = 0; side < 2; side++) {
+            const _Float16 *A = (side == 0 ? amat0 : amat1) + po;
+            r[side] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                *reinterpret_cast<const h8 *>(A + 8 * ((2 * s + h) ^ sw)), b[side].v[s], r[side],
+                0, 0, 0);
+          }
+        if (part == 0) {
+          r[0] *= kLoScale;
+          r[1] *= kLoScale;
+        }
       }
+#pragma unroll
+      for (uint32_t side = 0; side < 2; side++) {
+        if ((srcs[side] & SRC_TAG) == SRC_TIP && __builtin_amdgcn_ballot_w64(code[side] >= S)) {
+          const bool unk = code[side] >= S;
+#pragma unroll
+          for (uint32_t q = 0; q < 16; q++)
+            if (unk) r[side][q] = (int)row_of(0, 16 * rb + q) < lim ? 1.0f : 0.0f;
+        }
+      }
+      acc[rb] = r[0] * r[1];
+    }
+    if (more) {
+      lds_barrier();  // every wavefront is done with this node's matrices
+      store_mats();   // the next entry's
+      take();
+      if (i + 2 < n) fetch_mats(i + 2);
     }
     // product and per-pattern normalisation across the lane pair
     float v[32];
@@ -281,21 +307,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
       v[16 + r] = acc[1][r];
       mx = fmaxf(mx, fmaxf(v[r], v[16 + r]));
     }
-    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    mx = pair_max(mx);
     const int ex = mx > 0.0f ? __builtin_amdgcn_frexp_expf(mx) : 0;
 #pragma unroll
     for (uint32_t r = 0; r < 32; r++) v[r] = __builtin_amdgcn_ldexpf(v[r], kXTop - ex);
     E = ce[0] + ce[1] + ex - kXTop;
 
     if (more) {
+      Tile x;
 #pragma unroll
       for (uint32_t s = 0; s < 4; s++)
 #pragma unroll
-        for (uint32_t m = 0; m < 8; m++) x.v[s][m] = (_Float16)v[8 * s + m];
-      h8 *g = reinterpret_cast<h8 *>(tile_ptr(out));
+        for (uint32_t q = 0; q < 8; q++) x.v[s][q] = (_Float16)v[8 * s + q];
+      const uint4 en = ents[i + 1];
 #pragma unroll
-      for (uint32_t s = 0; s < 4; s++) g[s * 64u] = x.v[s];
-      if (h == 0) a.exps[exp_idx(out)] = (int16_t)E;
+      for (uint32_t side = 0; side < 2; side++)
+        if (((side == 0 ? en.y : en.z) & SRC_TAG) == SRC_PREV) {
+          y[side] = x;
+          ey[side] = E;
+        }
+      // streaming stores: keep the L2 for the matrices every workgroup reads
+      // (both halves of a pattern store the same exponent)
+      u4 *g = reinterpret_cast<u4 *>(tile_ptr(out));
+#pragma unroll
+      for (uint32_t s = 0; s < 4; s++)
+        __builtin_nontemporal_store(__builtin_bit_cast(u4, x.v[s]), g + s * 64u);
+      __builtin_nontemporal_store((int16_t)E, exp_ptr(out));
+      lds_barrier();  // the next entry's matrices are in LDS
     } else {
       // root of the update: ln(sum_j pi_j x_j) + E ln 2 in FP64 from the
       // unrounded FP32 values (the root partial is never read as a child)
@@ -303,22 +341,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
       const double *pl = pi_s + 4 * h;
 #pragma unroll
       for (uint32_t r = 0; r < 32; r++) sum = fma((double)v[r], pl[row_of(0, r)], sum);
-      sum += __shfl_xor(sum, 32);
+      sum = pair_sum(sum);
       if (p < a.P && h == 0) a.site[(size_t)c * a.P + p] = fma((double)E, LN2, log(sum));
     }
   }
 }
 
-hipError_t prepare_prune_codon(int max_lds_bytes) {
-  return hipFuncSetAttribute(reinterpret_cast<const void *>(&prune_codon_kernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes);
-}
+// the kernel uses static LDS only: nothing to raise
+hipError_t prepare_prune_codon(int) { return hipSuccess; }
 
 hipError_t launch_prune_codon(const CodonArgs &a, hipStream_t s) {
   if (a.n == 0 || a.Ppad % (32u * kWaves) || a.S > 64) return hipErrorInvalidValue;
-  const size_t lds = prune_codon_lds_bytes(a.n);
   dim3 grid(a.Ppad / (32u * kWaves), a.K);
-  hipLaunchKernelGGL(prune_codon_kernel, grid, dim3(256), (uint32_t)lds, s, a);
+  hipLaunchKernelGGL(prune_codon_kernel, grid, dim3(256), 0, s, a,
+                     reinterpret_cast<const uint4 *>(a.entries));
   return hipGetLastError();
 }
 
