@@ -72,6 +72,7 @@ struct b3h_calc {
   double *d_total = nullptr;
   void *d_pmats = nullptr;  // [I][2][K][S*S] float or double
   DevModel *d_model = nullptr;
+  double *d_ucodon = nullptr;  // MFMA path: [64][4096] eigenvector products
   bool model_set = false;
 
   // staging (pinned host -> device), double buffered
@@ -118,7 +119,7 @@ void release(b3h_calc *c) {
   void *ptrs[] = {c->d_codes, c->d_partials, c->d_exps, c->d_scales,
                   c->d_scale_sums, c->d_scale_sums_bak, c->d_site, c->d_mixed,
                   c->d_weights, c->d_blocksum, c->d_mix_blocksum,
-                  c->d_counter, c->d_total, c->d_pmats, c->d_model};
+                  c->d_counter, c->d_total, c->d_pmats, c->d_model, c->d_ucodon};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (int s = 0; s < kStages; s++) {
@@ -290,7 +291,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
   if (codon) {
     const _Float16 *amats = reinterpret_cast<const _Float16 *>(d + off_m);
     if (mode == 1) {
-      HIPCHK(launch_pbuild_codon(d_entries, n, K, d_times, d_rates, c->d_model,
+      HIPCHK(launch_pbuild_codon(d_entries, n, K, d_times, d_rates, c->d_model, c->d_ucodon,
                                  static_cast<_Float16 *>(c->d_pmats), c->stream));
       amats = static_cast<const _Float16 *>(c->d_pmats);
     }
@@ -560,6 +561,7 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   const size_t mat_bytes = codon ? 8192 * 2 : (size_t)S * S * (f32 ? 4 : 8);
   if ((st = dev_alloc(c, &c->d_pmats, (size_t)c->I * 2 * K * mat_bytes))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_model, sizeof(DevModel)))) return bail(st);
+  if (codon && (st = dev_alloc(c, (void **)&c->d_ucodon, (size_t)64 * 4096 * 8))) return bail(st);
 
   // staging: entries + host matrices (the larger of the f64 input and the
   // FP16 A layout) + times + rates + stationary frequencies
@@ -649,6 +651,7 @@ b3h_status b3h_set_model(b3h_calc *c, const double *q, const double *freqs,
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpyAsync(c->d_model, &hm, sizeof hm, hipMemcpyHostToDevice, c->stream));
   HIPCHK(launch_eigen(c->d_model, c->stream));
+  if (c->precision == B3H_F16_MFMA) HIPCHK(launch_ubuild_codon(c->d_model, c->d_ucodon, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   for (uint32_t i = 0; i < S; i++) c->freqs[i] = freqs[i];
   c->model_set = true;

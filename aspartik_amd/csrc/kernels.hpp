@@ -75,9 +75,12 @@ struct CodonArgs {
 size_t prune_codon_lds_bytes(uint32_t n);
 hipError_t prepare_prune_codon(int max_lds_bytes);
 hipError_t launch_prune_codon(const CodonArgs &a, hipStream_t s);
+// U[q][el] = V[j][q] V^-1[q][k] of the model (64 x 4096 doubles), once per
+// model; the matrices of a propose are then E U (prune_codon.hip)
+hipError_t launch_ubuild_codon(const DevModel *model, double *U, hipStream_t s);
 hipError_t launch_pbuild_codon(const Entry *entries, uint32_t n, uint32_t K, const double *times,
-                               const double *rates, const DevModel *model, _Float16 *out,
-                               hipStream_t s);
+                               const double *rates, const DevModel *model, const double *U,
+                               _Float16 *out, hipStream_t s);
 
 // ---- prune64.hip --------------------------------------------------------
 // FP64 plain baseline: one lane per (pattern, category), b3's e^scale_ln

@@ -359,47 +359,80 @@ hipError_t launch_prune_codon(const CodonArgs &a, hipStream_t s) {
 }
 
 // ---- transition matrices in A layout ------------------------------------
-// One workgroup per (entry, side, category): P = V diag(exp(lambda t)) V^-1 in
-// FP64 from the device eigensystem, split into FP16 hi and lo parts
-// (P = hi + 2^-20 lo) and stored with the contraction (column) index in the
-// kernel's order:
-//   A[part][j][16 s + 8 h + m] = part(P)[j][row(h, 8 s + m)],
-// zero outside the S states.
+// Every P(t) of a propose shares the eigenvectors: with
+//   U[q][el] = V[j][q] V^-1[q][k]   (el = 64 j + kslot, k = row(h, 8 s + m)
+//                                    for kslot = 16 s + 8 h + m; zero outside
+//                                    the S states)
+// built once per model, the matrices of a propose are one FP64 product
+//   P[mat][el] = sum_q E[mat][q] U[q][el],  E[mat][q] = exp(lambda_q t_mat),
+// split into FP16 hi and lo parts (P = hi + 2^-20 lo) in the kernel's A
+// layout: A[part][j][16 s + 8 h + m] = part(P)[j][row(h, 8 s + m)].
+
+// one workgroup per q (64), 256 threads x 16 elements
+__global__ __launch_bounds__(256) void ubuild_codon_kernel(const DevModel *model, double *U) {
+  const uint32_t q = blockIdx.x, S = (uint32_t)model->n;
+  for (uint32_t el = threadIdx.x; el < 4096u; el += blockDim.x) {
+    const uint32_t j = el / 64u, kslot = el % 64u;
+    const uint32_t k = row_of((kslot / 8u) & 1u, 8u * (kslot / 16u) + kslot % 8u);
+    U[(size_t)q * 4096u + el] =
+        (j < S && k < S && q < S) ? model->evec[j * S + q] * model->ievc[q * S + k] : 0.0;
+  }
+}
+
+hipError_t launch_ubuild_codon(const DevModel *model, double *U, hipStream_t s) {
+  hipLaunchKernelGGL(ubuild_codon_kernel, dim3(64), dim3(256), 0, s, model, U);
+  return hipGetLastError();
+}
+
+// grid (4096 / 256 element tiles, ceil(matrices / 16)): each thread owns one
+// element of 16 matrices; U is read once per workgroup, E from LDS.
 __global__ __launch_bounds__(256) void pbuild_codon_kernel(const Entry *entries, uint32_t n,
                                                            uint32_t K, const double *times,
                                                            const double *rates,
-                                                           const DevModel *model,
+                                                           const DevModel *model, const double *U,
                                                            _Float16 *out) {
-  __shared__ double ev[64];
-  const uint32_t mat = blockIdx.x;  // (entry * 2 + side) * K + c
-  const uint32_t c = mat % K, es = mat / K, side = es & 1u, e = es >> 1;
+  constexpr uint32_t kM = 16;
+  __shared__ double E[kM][64];
+  const uint32_t nmat = n * 2u * K, m0 = blockIdx.y * kM;
   const uint32_t S = (uint32_t)model->n;
-  const uint32_t orig = entries[e].meta >> 8;
-  const double t = times[2 * orig + side] * rates[c];
-  if (threadIdx.x < 64) ev[threadIdx.x] = threadIdx.x < S ? exp(model->eval[threadIdx.x] * t) : 0.0;
-  __syncthreads();
-  for (uint32_t el = threadIdx.x; el < 4096u; el += blockDim.x) {
-    const uint32_t j = el / 64u, kslot = el % 64u;
-    const uint32_t s = kslot / 16u, hh = (kslot / 8u) & 1u, m = kslot % 8u;
-    const uint32_t k = row_of(hh, 8 * s + m);
+  for (uint32_t w = threadIdx.x; w < kM * 64u; w += blockDim.x) {
+    const uint32_t mi = w / 64u, q = w % 64u, mat = m0 + mi;
     double v = 0.0;
-    if (j < S && k < S) {
-      for (uint32_t q = 0; q < S; q++)
-        v = fma(model->evec[j * S + q] * ev[q], model->ievc[q * S + k], v);
+    if (mat < nmat && q < S) {
+      const uint32_t c = mat % K, es = mat / K, side = es & 1u, e = es >> 1;
+      const uint32_t orig = entries[e].meta >> 8;
+      v = exp(model->eval[q] * (times[2 * orig + side] * rates[c]));
     }
+    E[mi][q] = v;
+  }
+  __syncthreads();
+  const uint32_t el = blockIdx.x * 256u + threadIdx.x;
+  double acc[kM];
+#pragma unroll
+  for (uint32_t mi = 0; mi < kM; mi++) acc[mi] = 0.0;
+  for (uint32_t q = 0; q < S; q++) {
+    const double u = U[(size_t)q * 4096u + el];
+#pragma unroll
+    for (uint32_t mi = 0; mi < kM; mi++) acc[mi] = fma(E[mi][q], u, acc[mi]);
+  }
+#pragma unroll
+  for (uint32_t mi = 0; mi < kM; mi++) {
+    const uint32_t mat = m0 + mi;
+    if (mat >= nmat) break;
+    const double v = acc[mi];
     const _Float16 hi = (_Float16)(float)v;
-    const _Float16 lo = (_Float16)(float)((v - (double)hi) * 0x1p20);
     out[(size_t)mat * 8192u + el] = hi;
-    out[(size_t)mat * 8192u + 4096u + el] = lo;
+    out[(size_t)mat * 8192u + 4096u + el] = (_Float16)(float)((v - (double)hi) * 0x1p20);
   }
 }
 
 hipError_t launch_pbuild_codon(const Entry *entries, uint32_t n, uint32_t K, const double *times,
-                               const double *rates, const DevModel *model, _Float16 *out,
-                               hipStream_t s) {
+                               const double *rates, const DevModel *model, const double *U,
+                               _Float16 *out, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(pbuild_codon_kernel, dim3(n * 2 * K), dim3(256), 0, s, entries, n, K, times,
-                     rates, model, out);
+  const uint32_t nmat = n * 2u * K;
+  hipLaunchKernelGGL(pbuild_codon_kernel, dim3(16, (nmat + 15) / 16), dim3(256), 0, s, entries, n,
+                     K, times, rates, model, U, out);
   return hipGetLastError();
 }
 

@@ -11,7 +11,7 @@ import numpy as np
 
 from . import sim
 from .b3.calculator import gamma_rates
-from .b3.substitution import GTR, HKY
+from .b3.substitution import GTR, GY94, HKY
 
 
 def gtr_gamma4(num_leaves=128, num_patterns=100_000, alpha=0.5, seed=4, weights=(1, 5)):
@@ -129,4 +129,34 @@ def hky_gamma4_batch(count=256, taxa=(50, 500), patterns=(2_000, 50_000), seed=4
         out.append(dict(parents=parents, children=children, heights=heights, kappa=kappa,
                         model=model, codes=sim.dna_codes(states), L=L, P=P))
     return dict(name=f"hky_gamma4_batch{count}", instances=out, K=4, alpha=alpha, rates=cat,
+                seed=seed)
+
+
+def gy94_codon(num_leaves=64, num_patterns=20_000, omega=0.3, kappa=2.0, alpha=0.5, seed=4):
+    """GY94 codon workload: omega 0.3, kappa 2, F3x4 frequencies from three
+    Dirichlet(10) draws, Gamma4 alpha 0.5, ``num_patterns`` unique codon
+    patterns simulated on a seeded Yule tree (multithreaded host simulator,
+    first-occurrence order), weights 1.  States are the 61 sense codons in
+    GY94's order."""
+    rng = np.random.default_rng(seed)
+    parents, children, heights = sim.yule_tree(num_leaves, rng, 1.0, 0.1)
+    model = GY94(omega, kappa, np.stack([rng.dirichlet([10.0] * 4) for _ in range(3)]))
+    K = 4
+    cat = gamma_rates(alpha, K)
+    pi = model.frequencies()
+    q = model.q_matrix()
+    seen, parts, have = set(), [], 0
+    batch = int(num_patterns * 1.05) + 64
+    for _ in range(100):
+        st = sim.simulate_states_fast(parents, children, heights, q, pi, cat, batch,
+                                      int(rng.integers(0, 2**62)))
+        keep = sim.unique_columns(st, seen)[: num_patterns - have]
+        parts.append(st[:, keep])
+        have += keep.size
+        if have >= num_patterns:
+            break
+    codes = np.ascontiguousarray(np.concatenate(parts, axis=1)[:, :num_patterns])
+    return dict(name=f"gy94_codon_{num_leaves}taxa_{num_patterns}pat", parents=parents,
+                children=children, heights=heights, model=model, codes=codes,
+                weights=np.ones(num_patterns, np.uint32), K=K, alpha=alpha, rates=cat,
                 seed=seed)
