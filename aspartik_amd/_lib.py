@@ -14,6 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libb3hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "b3hip.h")
+MCMC_HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "b3mcmc.h")
 
 B3H_OK = 0
 STATUS_NAMES = {
@@ -69,12 +70,49 @@ class Update(C.Structure):
     ]
 
 
+class ChainConfig(C.Structure):
+    """b3m_chain_config (include/b3mcmc.h)"""
+    _fields_ = [
+        ("device", C.c_int32),
+        ("categories", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("birth_rate", C.c_double),
+        ("alpha_prior_rate", C.c_double),
+        ("w_node_slide", C.c_double),
+        ("w_tree_scale", C.c_double),
+        ("w_rates", C.c_double),
+        ("w_alpha", C.c_double),
+        ("w_narrow", C.c_double),
+        ("scale_factor", C.c_double),
+        ("delta_tuning", C.c_double),
+    ]
+
+
+class RunStats(C.Structure):
+    """b3m_run_stats (include/b3mcmc.h)"""
+    _fields_ = [
+        ("steps", C.c_uint64), ("accepted", C.c_uint64), ("exact_evals", C.c_uint64),
+        ("surrogate_evals", C.c_uint64), ("stage1_rejects", C.c_uint64),
+        ("stage2_rejects", C.c_uint64), ("aborts", C.c_uint64), ("prior_rejects", C.c_uint64),
+        ("op_proposed", C.c_uint64 * 5), ("op_accepted", C.c_uint64 * 5),
+        ("collected", C.c_uint64), ("seconds", C.c_double),
+        ("likelihood_seconds", C.c_double), ("surrogate_seconds", C.c_double),
+    ]
+
+
+B3M_PLAIN = 0
+B3M_DELAYED = 1
+B3M_TRACE_COLS = 13
+TRACE_COLUMNS = ["step", "posterior", "likelihood", "prior", "root_height", "tree_length",
+                 "alpha", "rate_ac", "rate_ag", "rate_at", "rate_cg", "rate_ct", "rate_gt"]
+
 B3H_BATCH_SINGLE_SLOT = 1
 B3H_BATCH_GRAPH = 1
 
 u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 
 _lib = None
 
@@ -114,6 +152,36 @@ SIGNATURES = {
     "b3h_batch_device_bytes": (C.c_uint64, [C.c_void_p]),
 }
 
+# every symbol declared in include/b3mcmc.h
+MCMC_SIGNATURES = {
+    "b3m_ess": (C.c_double, [f64p, C.c_size_t]),
+    "b3m_mlp_create": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint64,
+                                 C.POINTER(C.c_void_p)]),
+    "b3m_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "b3m_mlp_set_scaling": (C.c_int, [C.c_void_p, f32p, f32p, C.c_float, C.c_float]),
+    "b3m_mlp_forward": (C.c_int, [C.c_void_p, f32p, C.c_uint32, f32p]),
+    "b3m_mlp_num_params": (C.c_size_t, [C.c_void_p]),
+    "b3m_mlp_loss_grad": (C.c_int, [C.c_void_p, f32p, f32p, C.c_uint32, C.POINTER(C.c_float),
+                                    f32p]),
+    "b3m_mlp_get_params": (C.c_int, [C.c_void_p, f32p]),
+    "b3m_mlp_set_params": (C.c_int, [C.c_void_p, f32p]),
+    "b3m_mlp_train": (C.c_int, [C.c_void_p, f32p, f32p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                C.c_float, C.POINTER(C.c_float)]),
+    "b3m_chain_config_default": (None, [C.POINTER(ChainConfig)]),
+    "b3m_chain_create": (C.c_int, [C.POINTER(ChainConfig), u8p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                   u32p, u32p, f64p, f64p, f64p, C.c_double,
+                                   C.POINTER(C.c_void_p)]),
+    "b3m_chain_destroy": (C.c_int, [C.c_void_p]),
+    "b3m_chain_num_features": (C.c_uint32, [C.c_void_p]),
+    "b3m_chain_features": (C.c_int, [C.c_void_p, f32p]),
+    "b3m_chain_run": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p,
+                                C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(RunStats)]),
+    "b3m_chain_set_surrogate": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "b3m_chain_params": (C.c_int, [C.c_void_p, u32p, u32p, f64p, f64p, C.POINTER(C.c_double)]),
+    "b3m_chain_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]),
+}
+
 
 def lib():
     """Load libb3hip.so; raises if it is not built."""
@@ -125,7 +193,7 @@ def lib():
             f"{LIB_PATH} is not built; run __graft_entry__.build() "
             "(the HIP library is required: there is no CPU fallback)")
     L = C.CDLL(LIB_PATH)
-    for name, (res, args) in SIGNATURES.items():
+    for name, (res, args) in {**SIGNATURES, **MCMC_SIGNATURES}.items():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args

@@ -3,11 +3,12 @@ tree, substitution and likelihood objects that drive the pruning path)."""
 from .batch import Batch
 from .calculator import Calculator, HipCalculator, gamma_rates
 from .likelihood import DNALikelihood, GammaLikelihood, deduplicate
+from .mcmc import Chain, Surrogate, ess
 from .substitution import GTR, GY94, HKY, JC, K80, SENSE_CODONS
 from .tree import TOMBSTONE, DatedTree
 
 __all__ = [
     "Batch", "Calculator", "HipCalculator", "gamma_rates", "DNALikelihood", "GammaLikelihood",
     "deduplicate", "GTR", "GY94", "HKY", "JC", "K80", "SENSE_CODONS", "TOMBSTONE",
-    "DatedTree",
+    "DatedTree", "Chain", "Surrogate", "ess",
 ]

@@ -10,10 +10,10 @@ import pytest
 from aspartik_amd import _lib
 
 
-def _declared():
-    text = open(_lib.HEADER_PATH).read()
+def _declared(path=None, prefix="b3h"):
+    text = open(path or _lib.HEADER_PATH).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(b3h_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_declares_expected_entry_points():
@@ -33,6 +33,31 @@ def test_library_exports_every_declared_symbol():
         assert name in exported, name
         getattr(lib, name)
     assert set(_lib.SIGNATURES) == set(_declared())
+    mcmc = _declared(_lib.MCMC_HEADER_PATH, "b3m")
+    exported_m = set(re.findall(r"\bT (b3m_\w+)", out))
+    for name in mcmc:
+        assert name in exported_m, name
+        getattr(lib, name)
+    assert set(_lib.MCMC_SIGNATURES) == set(mcmc)
+
+
+def test_mcmc_struct_layouts_match_the_header():
+    """ctypes mirrors of b3m_chain_config / b3m_run_stats have the C sizes"""
+    src = _lib.MCMC_HEADER_PATH
+    prog = ("#include <stdio.h>\n#include \"" + src + "\"\n"
+            "int main(void){printf(\"%zu %zu\\n\", sizeof(b3m_chain_config),"
+            " sizeof(b3m_run_stats));return 0;}\n")
+    import os
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        open(c, "w").write(prog)
+        exe = os.path.join(d, "t")
+        subprocess.run(["gcc", "-I", os.path.dirname(src), "-o", exe, c], check=True)
+        a, b = map(int, subprocess.run([exe], capture_output=True, text=True,
+                                       check=True).stdout.split())
+    assert a == C.sizeof(_lib.ChainConfig)
+    assert b == C.sizeof(_lib.RunStats)
 
 
 def test_create_reports_missing_device_as_status():

@@ -1,0 +1,177 @@
+"""MCMC driver, MLP surrogate and delayed acceptance on the GPU.
+
+* The surrogate's forward pass and gradients (HIP kernels, BF16 weights,
+  FP32 accumulation) against a float64 PyTorch-CPU model holding the same
+  BF16-rounded weights; the tolerance (2e-3 relative, in norm) covers FP32
+  accumulation over <= 404 terms.
+* The chain's ln L after many accepted and rejected moves (partial updates,
+  model restores, topology moves) against the FP64 oracle for the chain's
+  current state, and its log prior against the formula.
+* Delayed acceptance keeps the posterior: posterior means of a plain chain
+  and of delayed-acceptance chains (trained and untrained surrogate) agree
+  within 5 Monte Carlo standard errors (ESS-based).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from aspartik_amd import sim
+from aspartik_amd.b3 import GTR, Chain, Surrogate, ess, gamma_rates
+from oracle import oracle as O
+from tests.helpers import random_tree
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _bf16(a):
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def _torch_ref(s, p, x, t):
+    W1, b1, W2, b2, w3, b3 = s.unpack(p)
+    T = lambda a: torch.tensor(np.asarray(a, np.float64), dtype=torch.float64,  # noqa: E731
+                               requires_grad=True)
+    W1t, b1t, W2t, b2t, w3t, b3t = (T(_bf16(W1)), T(b1), T(_bf16(W2)), T(b2), T(_bf16(w3)), T(b3))
+    xt = torch.tensor(np.asarray(x, np.float64))
+    gelu = lambda z: torch.nn.functional.gelu(z, approximate="tanh")  # noqa: E731
+    y = gelu(gelu(xt @ W1t + b1t) @ W2t + b2t) @ w3t + b3t
+    loss = ((y - torch.tensor(np.asarray(t, np.float64))) ** 2).mean()
+    loss.backward()
+    g = np.concatenate([v.grad.numpy().reshape(-1) for v in (W1t, b1t, W2t, b2t, w3t, b3t)])
+    return y.detach().numpy(), float(loss), g
+
+
+@pytest.mark.parametrize("D,H,B", [(404, 256, 64), (37, 64, 5), (404, 256, 1)])
+def test_surrogate_forward_and_gradients(hip_device, D, H, B):
+    s = Surrogate(D, H, seed=D + B)
+    rng = np.random.default_rng(B)
+    p = s.params()
+    p[s.num_params - H - 1:] = rng.normal(size=H + 1).astype(np.float32) * 0.3  # nonzero output
+    p[D * H:D * H + H] = rng.normal(size=H).astype(np.float32) * 0.1            # and biases
+    s.set_params(p)
+    x = rng.normal(size=(B, D)).astype(np.float32)
+    t = rng.normal(size=B).astype(np.float32)
+    y_ref, loss_ref, g_ref = _torch_ref(s, s.params(), x, t)
+    y = s.forward(x)
+    np.testing.assert_allclose(y, y_ref, rtol=2e-3, atol=2e-3 * np.abs(y_ref).max())
+    loss, g = s.loss_grad(x, t)
+    assert abs(loss - loss_ref) <= 2e-3 * loss_ref
+    assert np.linalg.norm(g - g_ref) <= 2e-3 * np.linalg.norm(g_ref)
+    # each block separately (a transposed or misplaced block fails here)
+    for got, ref in zip(s.unpack(g), s.unpack(g_ref)):
+        assert np.linalg.norm(got - ref) <= 5e-3 * np.linalg.norm(ref) + 1e-7
+
+
+def test_surrogate_gradient_is_a_derivative(hip_device):
+    """Finite differences of the reference loss pin the reference gradient."""
+    s = Surrogate(11, 16, seed=3)
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(7, 11)).astype(np.float32)
+    t = rng.normal(size=7).astype(np.float32)
+    p = s.params().astype(np.float64)
+    _, _, g = _torch_ref(s, p, x, t)
+    # biases are used unrounded, so the loss is differentiable in them
+    b1, b2 = 11 * 16, 11 * 16 + 16 + 16 * 16
+    for idx in [b1, b1 + 9, b2 + 2, b2 + 15, len(p) - 1]:
+        e = np.zeros_like(p)
+        e[idx] = 1e-4
+        lp = _torch_ref(s, p + e, x, t)[1]
+        lm = _torch_ref(s, p - e, x, t)[1]
+        assert abs((lp - lm) / 2e-4 - g[idx]) < 1e-5 + 1e-3 * abs(g[idx])
+
+
+def test_surrogate_learns_a_smooth_function(hip_device):
+    rng = np.random.default_rng(5)
+    D, n = 16, 12288
+    a = rng.normal(size=D)
+    X = rng.normal(size=(n + 2048, D)).astype(np.float32)
+    y = (X @ a - 0.5 * (X ** 2).sum(1) + 3.0).astype(np.float32)
+    s = Surrogate(D, 128, seed=9)
+    s.fit_scaling(X[:n], y[:n])
+    before = np.mean((s.forward(X[n:]) - y[n:]) ** 2)
+    loss = s.train(X[:n], y[:n], epochs=60, batch=256, lr=2e-3)
+    after = np.mean((s.forward(X[n:]) - y[n:]) ** 2)
+    r2 = 1 - after / np.var(y[n:])
+    assert loss < 0.05 and after < 0.05 * before and r2 > 0.95, (loss, before, after, r2)
+
+
+def _data(L, sites, seed, alpha=0.5):
+    rng = np.random.default_rng(seed)
+    par, ch, h = random_tree(L, rng, mean_branch=0.1)
+    freqs = rng.dirichlet([10] * 4)
+    rates = rng.dirichlet([2] * 6)
+    model = GTR(freqs, rates)
+    codes = sim.dna_codes(sim.simulate_states_fast(par, ch, h, model.q_matrix(), freqs,
+                                                   gamma_rates(alpha, 4), sites,
+                                                   int(rng.integers(0, 2**62))))
+    return dict(codes=codes, parents=par, children=ch, heights=h, rates=rates, freqs=freqs,
+                alpha=alpha, L=L)
+
+
+def _oracle_loglik(d, chain):
+    par, ch, h, r, a = chain.params()
+    t = O.Tree(d["L"], par, ch, h)
+    lik = O.GammaLikelihood(t, O.Model.gtr(r, d["freqs"]), d["codes"], d["codes"].shape[1],
+                            np.ones(d["codes"].shape[1], np.uint32), num_categories=4, alpha=a)
+    return lik.likelihood(), par, h, a
+
+
+def test_chain_loglik_matches_oracle_after_many_moves(hip_device):
+    d = _data(12, 600, 1)
+    c = Chain(d["codes"], d["parents"], d["children"], d["heights"], d["rates"], d["freqs"],
+              d["alpha"], birth_rate=2.0, seed=11)
+    for rep in range(4):
+        trace, stats, _ = c.run(150)
+        ll, lp, post = c.state()
+        ref, par, h, a = _oracle_loglik(d, c)
+        assert abs(ll - ref) <= 1e-6 * abs(ref), (rep, ll, ref)
+        # log prior: Yule (n - 1) ln lambda - lambda * tree length + Exp(1) on alpha
+        nonroot = par != 0xFFFFFFFF
+        tl = float((h[par[nonroot].astype(np.int64)] - h[nonroot]).sum())
+        assert abs(lp - ((d["L"] - 1) * math.log(2.0) - 2.0 * tl - a)) < 1e-9
+        assert abs(post - ll - lp) < 1e-9
+        assert stats["accepted"] > 0 and stats["exact_evals"] > 0
+        assert trace[-1, 2] == ll
+    assert all(n > 0 for n in stats["op_proposed"])
+
+
+def _mean_and_se(x):
+    e = max(ess(x), 2.0)
+    return x.mean(), x.std() / math.sqrt(e)
+
+
+def test_delayed_acceptance_keeps_the_posterior(hip_device):
+    d = _data(8, 400, 2)
+    kw = dict(birth_rate=2.0, w_narrow=1.0)
+    burn = Chain(d["codes"], d["parents"], d["children"], d["heights"], d["rates"], d["freqs"],
+                 d["alpha"], seed=5, **kw)
+    _, _, (X, y) = burn.run(3000, collect=3000)
+    par, ch, h, r, a = burn.params()
+    s = Surrogate(burn.num_features, 128, seed=1)
+    s.fit_scaling(X, y)
+    s.train(X, y, epochs=200, batch=256, lr=1e-3)
+    untrained = Surrogate(burn.num_features, 128, seed=2)
+    untrained.fit_scaling(X, y)
+    cols = {"tree_length": 5, "alpha": 6, "root_height": 4}
+    runs = {}
+    for name, sur, mode, seed in [("plain", None, "plain", 21), ("delayed", s, "delayed", 22),
+                                  ("untrained", untrained, "delayed", 23)]:
+        c = Chain(d["codes"], par, ch, h, r, d["freqs"], a, seed=seed, **kw)
+        if sur is not None:
+            c.set_surrogate(sur)
+        trace, stats, _ = c.run(20000, mode=mode)
+        runs[name] = (trace, stats)
+    assert runs["delayed"][1]["stage1_rejects"] > 0
+    assert runs["delayed"][1]["exact_evals"] < runs["plain"][1]["exact_evals"]
+    for col_name, col in cols.items():
+        m0, s0 = _mean_and_se(runs["plain"][0][:, col])
+        for other in ("delayed", "untrained"):
+            m1, s1 = _mean_and_se(runs[other][0][:, col])
+            z = abs(m0 - m1) / math.hypot(s0, s1)
+            assert z < 5.0, (col_name, other, m0, m1, s0, s1)
