@@ -84,7 +84,11 @@ class ChainConfig(C.Structure):
         ("w_alpha", C.c_double),
         ("w_narrow", C.c_double),
         ("scale_factor", C.c_double),
+        ("alpha_factor", C.c_double),
         ("delta_tuning", C.c_double),
+        ("tune_every", C.c_uint32),
+        ("tune_cutoff", C.c_uint64),
+        ("tune_magnitude", C.c_double),
     ]
 
 
@@ -97,6 +101,7 @@ class RunStats(C.Structure):
         ("op_proposed", C.c_uint64 * 5), ("op_accepted", C.c_uint64 * 5),
         ("collected", C.c_uint64), ("seconds", C.c_double),
         ("likelihood_seconds", C.c_double), ("surrogate_seconds", C.c_double),
+        ("tuning", C.c_double * 5),
     ]
 
 

@@ -182,10 +182,10 @@ class Chain:
             self._h, steps, m, int(record_every), trace.ctypes.data_as(C.c_void_p),
             None if cx is None else cx.ctypes.data_as(C.c_void_p),
             None if cy is None else cy.ctypes.data_as(C.c_void_p), int(collect), C.byref(st)))
-        stats = {f: getattr(st, f) for f, _ in _lib.RunStats._fields_
-                 if f not in ("op_proposed", "op_accepted")}
-        stats["op_proposed"] = list(st.op_proposed)
-        stats["op_accepted"] = list(st.op_accepted)
+        arrays = ("op_proposed", "op_accepted", "tuning")
+        stats = {f: getattr(st, f) for f, _ in _lib.RunStats._fields_ if f not in arrays}
+        for f in arrays:
+            stats[f] = list(getattr(st, f))
         data = None
         if collect:
             n = int(st.collected)

@@ -230,6 +230,10 @@ struct b3m_chain {
   std::vector<uint32_t> internals, kids;
   std::vector<double> times;
   std::vector<float> feat;
+  // operator tuning (b3's Scheduler::tune)
+  double tree_scale_f = 0.75, alpha_f = 0.75, delta_t = 0.75;
+  uint64_t life_steps = 0;
+  uint64_t tune_acc[5] = {0}, tune_rej[5] = {0};
 };
 
 namespace {
@@ -303,7 +307,6 @@ void features(const b3m_chain *c, float *out) {
 double propose(b3m_chain *c, int op) {
   Tree &t = c->tree;
   Rng &rng = c->rng;
-  const double f = c->cfg.scale_factor;
   switch (op) {
     case kNodeSlide: {  // _node_slide.py:26-39
       if (t.L < 3) return -INFINITY;
@@ -316,12 +319,13 @@ double propose(b3m_chain *c, int op) {
       return 0.0;
     }
     case kTreeScale: {  // _tree_scale.py:27-37, tree.rs:335-348
+      const double f = c->tree_scale_f;
       const double s = f + (1.0 / f - f) * rng.uniform();
       for (uint32_t v = t.L; v < t.N; v++) t.set_height(v, t.heights[v] * s);
       return std::log(s) * ((double)(t.L - 1) - 2.0);
     }
     case kRates: {  // _delta_exchange.py:43-60
-      const double delta = rng.uniform() * (1.0 - c->cfg.delta_tuning) / 10.0;
+      const double delta = rng.uniform() * (1.0 - c->delta_t) / 10.0;
       const uint32_t d1 = rng.below(6);
       uint32_t d2 = rng.below(5);
       if (d2 >= d1) d2++;
@@ -330,6 +334,7 @@ double propose(b3m_chain *c, int op) {
       return 0.0;
     }
     case kAlpha: {  // _param_scale.py:32-39
+      const double f = c->alpha_f;
       const double s = f + (1.0 / f - f) * rng.uniform();
       c->alpha *= s;
       return -std::log(s);
@@ -376,6 +381,24 @@ void finalize(b3m_chain *c, bool accept) {  // mcmc.rs:362-393
   }
 }
 
+// Scheduler::tune (operators/mod.rs:164-190,467-482) with LinearOptimizer's
+// rate (optimizers.py:30-38)
+void tune(b3m_chain *c) {
+  const b3m_chain_config &g = c->cfg;
+  const uint64_t step = c->life_steps;
+  if (g.tune_every == 0 || step == 0 || step % g.tune_every != 0 || step >= g.tune_cutoff) return;
+  const double ratio = (double)step / (double)g.tune_cutoff;
+  const double rate = ratio < 0.9 ? g.tune_magnitude : g.tune_magnitude * 10.0 * (1.0 - ratio);
+  double *param[5] = {nullptr, &c->tree_scale_f, &c->delta_t, &c->alpha_f, nullptr};
+  for (int op = 0; op < kNumOps; op++) {
+    const double n = (double)(c->tune_acc[op] + c->tune_rej[op]);
+    if (!param[op] || n == 0.0) continue;
+    const double acc = (double)c->tune_acc[op] / n;
+    *param[op] = std::min(0.99, std::max(0.01, *param[op] + rate * (0.234 - acc)));
+    c->tune_acc[op] = c->tune_rej[op] = 0;
+  }
+}
+
 double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
       .count();
@@ -399,7 +422,11 @@ void b3m_chain_config_default(b3m_chain_config *cfg) {
   cfg->w_alpha = 1.0;
   cfg->w_narrow = 2.0;
   cfg->scale_factor = 0.75;
+  cfg->alpha_factor = 0.0;
   cfg->delta_tuning = 0.75;
+  cfg->tune_every = 0;
+  cfg->tune_cutoff = 0;
+  cfg->tune_magnitude = 0.5;
 }
 
 b3h_status b3m_chain_create(const b3m_chain_config *cfg, const uint8_t *codes, uint32_t L,
@@ -414,6 +441,7 @@ b3h_status b3m_chain_create(const b3m_chain_config *cfg, const uint8_t *codes, u
     return fail(B3H_ERR_ARG, "categories must be 1..64");
   if (!(cfg->birth_rate > 0.0) || !(cfg->alpha_prior_rate > 0.0) || !(alpha > 0.0) ||
       !(cfg->scale_factor > 0.0 && cfg->scale_factor < 1.0) ||
+      !(cfg->alpha_factor >= 0.0 && cfg->alpha_factor < 1.0) ||
       !(cfg->delta_tuning >= 0.0 && cfg->delta_tuning < 1.0))
     return fail(B3H_ERR_ARG, "bad prior or operator setting");
   const double wsum =
@@ -442,6 +470,9 @@ b3h_status b3m_chain_create(const b3m_chain_config *cfg, const uint8_t *codes, u
   for (int i = 0; i < 6; i++) c->rates[i] = c->rates_acc[i] = gtr_rates[i] / rsum;
   for (int i = 0; i < 4; i++) c->freqs[i] = freqs[i];
   c->alpha = c->alpha_acc = alpha;
+  c->tree_scale_f = cfg->scale_factor;
+  c->alpha_f = cfg->alpha_factor > 0.0 ? cfg->alpha_factor : cfg->scale_factor;
+  c->delta_t = cfg->delta_tuning;
   c->feat.resize(2 * (size_t)L + 4);
   b3h_config hc;
   b3h_config_default(&hc);
@@ -616,8 +647,13 @@ b3h_status b3m_chain_run(b3m_chain *c, uint64_t steps, uint32_t mode, uint32_t r
     if (accept) {
       s.accepted++;
       s.op_accepted[op]++;
+      c->tune_acc[op]++;
+    } else {
+      c->tune_rej[op]++;
     }
     s.steps++;
+    c->life_steps++;
+    tune(c);
     if (trace && step % record_every == 0) {
       double *row = trace + (step / record_every) * B3M_TRACE_COLS;
       row[B3M_COL_STEP] = (double)step;
@@ -633,6 +669,10 @@ b3h_status b3m_chain_run(b3m_chain *c, uint64_t steps, uint32_t mode, uint32_t r
   // a surrogate that was not refreshed after a topology change is stale
   if (mode != B3M_DELAYED) c->surrogate_valid = false;
   s.seconds = now() - t_start;
+  s.tuning[kNodeSlide] = s.tuning[kNarrow] = -1.0;
+  s.tuning[kTreeScale] = c->tree_scale_f;
+  s.tuning[kRates] = c->delta_t;
+  s.tuning[kAlpha] = c->alpha_f;
   if (stats) *stats = s;
   return st;
 }

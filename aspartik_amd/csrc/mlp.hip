@@ -196,6 +196,44 @@ __global__ void k_adam(float *p, const float *g, float *m, float *v, uint16_t *p
   pb[i] = to_bf16(q);
 }
 
+// One sample, one workgroup: the whole forward pass with the input passed by
+// value (no copy to the device) and the result written to pinned host
+// memory (no copy back): one launch per surrogate call in the MCMC loop.
+// The hidden units use the same FP32 summation order as k_dense_fwd.
+constexpr uint32_t kMaxFusedD = 512, kMaxFusedH = 1024;
+struct FusedInput {
+  float x[kMaxFusedD];
+};
+__global__ __launch_bounds__(256) void k_predict1(FusedInput in, uint32_t D, uint32_t H,
+                                                  const float *xmean, const float *xstd,
+                                                  const uint16_t *W1, const float *b1,
+                                                  const uint16_t *W2, const float *b2,
+                                                  const uint16_t *w3, const float *b3, float ymean,
+                                                  float ystd, float *out) {
+  __shared__ float xs[kMaxFusedD];
+  __shared__ float h1[kMaxFusedH];
+  __shared__ float part[4];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t k = tid; k < D; k += 256u) xs[k] = (in.x[k] - xmean[k]) / xstd[k];
+  __syncthreads();
+  for (uint32_t n = tid; n < H; n += 256u) {
+    float z = b1[n];
+    for (uint32_t k = 0; k < D; k++) z = fmaf(xs[k], from_bf16(W1[(size_t)k * H + n]), z);
+    h1[n] = gelu(z);
+  }
+  __syncthreads();
+  float s = 0.0f;
+  for (uint32_t n = tid; n < H; n += 256u) {
+    float z = b2[n];
+    for (uint32_t k = 0; k < H; k++) z = fmaf(h1[k], from_bf16(W2[(size_t)k * H + n]), z);
+    s = fmaf(gelu(z), from_bf16(w3[n]), s);
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((tid & 63u) == 0) part[tid >> 6] = s;
+  __syncthreads();
+  if (tid == 0) out[0] = (part[0] + part[1] + part[2] + part[3] + b3[0]) * ystd + ymean;
+}
+
 __global__ void k_cast(const float *p, uint16_t *pb, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) pb[i] = to_bf16(p[i]);
@@ -217,6 +255,7 @@ struct b3m_mlp {
         *T = nullptr, *dY = nullptr, *dZ1 = nullptr, *dZ2 = nullptr, *loss = nullptr;
   float *xin = nullptr;
   uint32_t *idx = nullptr;
+  float *h_out = nullptr;  // pinned result of the fused single-sample pass
   uint64_t adam_t = 0, seed = 0;
   std::vector<void *> owned;
 };
@@ -302,6 +341,17 @@ hipError_t backward(b3m_mlp *m, uint32_t B) {
 // forward pass for the chain's use: x (host, B rows) -> y (host)
 namespace b3h {
 b3h_status mlp_predict(b3m_mlp *m, const float *x, uint32_t B, float *y) {
+  if (B == 1 && m->D <= kMaxFusedD && m->H <= kMaxFusedH && m->h_out) {
+    FusedInput in;
+    std::memcpy(in.x, x, (size_t)m->D * 4);
+    hipLaunchKernelGGL(k_predict1, dim3(1), dim3(256), 0, m->stream, in, m->D, m->H, m->xmean,
+                       m->xstd, m->pb + m->oW1, m->p + m->ob1, m->pb + m->oW2, m->p + m->ob2,
+                       m->pb + m->ow3, m->p + m->ob3, m->ymean, m->ystd, m->h_out);
+    MCHK(hipGetLastError());
+    MCHK(hipStreamSynchronize(m->stream));
+    y[0] = *(volatile float *)m->h_out;
+    return B3H_OK;
+  }
   b3h_status st = ensure_batch(m, B);
   if (st) return st;
   MCHK(hipMemcpyAsync(m->xin, x, (size_t)B * m->D * 4, hipMemcpyHostToDevice, m->stream));
@@ -379,6 +429,7 @@ b3h_status b3m_mlp_create(int device, uint32_t inputs, uint32_t hidden, uint64_t
                        m->p, m->pb, m->np);
     ok = hipStreamSynchronize(m->stream) == hipSuccess;
   }
+  if (ok) ok = hipHostMalloc((void **)&m->h_out, 64, hipHostMallocDefault) == hipSuccess;
   if (!ok) return fail(b3h::host_fail(B3H_ERR_HIP, "surrogate initialisation failed"));
   *out = m;
   return B3H_OK;
@@ -389,6 +440,7 @@ b3h_status b3m_mlp_destroy(b3m_mlp *m) {
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   for (void *p : m->owned) (void)hipFree(p);
+  if (m->h_out) (void)hipHostFree(m->h_out);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return B3H_OK;

@@ -160,3 +160,23 @@ def gy94_codon(num_leaves=64, num_patterns=20_000, omega=0.3, kappa=2.0, alpha=0
                 children=children, heights=heights, model=model, codes=codes,
                 weights=np.ones(num_patterns, np.uint32), K=K, alpha=alpha, rates=cat,
                 seed=seed)
+
+
+def gtr_gamma4_mcmc(num_leaves=200, num_sites=200_000, alpha=0.5, seed=4):
+    """MCMC workload: GTR rates ~ Dirichlet(1), frequencies ~ Dirichlet(10),
+    Gamma4 alpha 0.5, ``num_sites`` sites simulated on a seeded Yule tree
+    (multithreaded host simulator).  With 200 taxa duplicate columns are
+    vanishingly rare, so the sites are the patterns (weights 1)."""
+    rng = np.random.default_rng(seed)
+    parents, children, heights = sim.yule_tree(num_leaves, rng, 1.0, 0.1)
+    freqs = rng.dirichlet([10.0] * 4)
+    rates6 = rng.dirichlet([1.0] * 6)
+    model = GTR(freqs, rates6)
+    K = 4
+    cat = gamma_rates(alpha, K)
+    states = sim.simulate_states_fast(parents, children, heights, model.q_matrix(), freqs, cat,
+                                      num_sites, int(rng.integers(0, 2**62)))
+    return dict(name=f"gtr_gamma4_{num_leaves}taxa_{num_sites}pat_mcmc", parents=parents,
+                children=children, heights=heights, model=model, codes=sim.dna_codes(states),
+                weights=np.ones(num_sites, np.uint32), K=K, alpha=alpha, rates=cat,
+                gtr_rates=rates6, freqs=freqs, seed=seed)

@@ -14,6 +14,7 @@ throughput on the host cores (cpu_baseline).
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload batch [--instances 256]   (batched HKY+Gamma4)
        python bench.py --workload codon                     (GY94 61-state, MFMA)
+       python bench.py --workload mcmc                      (ESS/s, delayed acceptance)
 """
 from __future__ import annotations
 
@@ -253,6 +254,124 @@ def bench_codon(args):
     }), flush=True)
 
 
+def bench_mcmc(args):
+    """ESS per second of delayed-acceptance MH against plain MH: GTR+Gamma4,
+    200 taxa, 200,000 sites.  Burn-in with plain MH collects (features,
+    exact ln L) pairs; the MLP surrogate (404 -> 256 -> 256 -> 1) is trained
+    on them; then plain and delayed-acceptance chains run the same number of
+    steps from the burn-in's final state.  ESS is Geyer's estimator on the
+    log posterior, tree length and alpha traces; ESS/s uses each chain's
+    wall clock (likelihood, surrogate and host work)."""
+    from aspartik_amd import workloads
+    from aspartik_amd.b3 import Chain, Surrogate, ess
+
+    log("[bench] building MCMC workload")
+    wl = workloads.gtr_gamma4_mcmc(args.mcmc_taxa, args.mcmc_sites)
+    L, P = args.mcmc_taxa, args.mcmc_sites
+    kw = dict(birth_rate=float((L - 1) / (wl["heights"][wl["parents"][:-1].astype(np.int64)]
+                                          - wl["heights"][:-1]).sum()))
+    # operator tuning during burn-in (b3's LinearOptimizer schedule), frozen after
+    burn = Chain(wl["codes"], wl["parents"], wl["children"], wl["heights"], wl["gtr_rates"],
+                 wl["freqs"], wl["alpha"], seed=101, tune_every=max(50, args.burnin // 40),
+                 tune_cutoff=args.burnin, **kw)
+    t0 = time.perf_counter()
+    _, bstats, (X, y) = burn.run(args.burnin, collect=args.burnin)
+    log(f"[bench] burn-in {args.burnin} steps: {time.perf_counter() - t0:.1f} s, "
+        f"{len(y)} exact evaluations collected")
+    state = burn.params()
+    nf = burn.num_features
+    tuned = dict(scale_factor=bstats["tuning"][1], delta_tuning=bstats["tuning"][2],
+                 alpha_factor=bstats["tuning"][3])
+    kw.update(tuned)
+    burn.close()
+    # the surrogate must resolve differences near the posterior's bulk; far
+    # below it, any value that rejects will do: clip the targets there
+    y = np.maximum(y, y.max() - args.clip)
+    # surrogate: train on 90 %, report held-out error on 10 %
+    order = np.random.default_rng(7).permutation(len(y))
+    ntr = int(0.9 * len(y))
+    tr, te = order[:ntr], order[ntr:]
+    sur = Surrogate(nf, 256, seed=3)
+    sur.fit_scaling(X[tr], y[tr])
+    t0 = time.perf_counter()
+    tr_loss = sur.train(X[tr], y[tr], epochs=args.epochs, batch=256, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    pred = sur.forward(X[te])
+    top = te[y[te] > y.max() - args.clip / 4]
+    held = {"rmse_loglik": float(np.sqrt(np.mean((pred - y[te]) ** 2))),
+            "rmse_top_quarter": float(np.sqrt(np.mean((sur.forward(X[top]) - y[top]) ** 2)))
+            if top.size else None,
+            "loglik_std": float(np.std(y[te])), "train_loss_standardised": tr_loss,
+            "train_pairs": int(ntr), "train_seconds": t_train, "clip": args.clip,
+            "tuned_operators": tuned}
+    log(f"[bench] surrogate trained in {t_train:.1f} s; held-out RMSE {held['rmse_loglik']:.2f} "
+        f"(ln L std {held['loglik_std']:.2f})")
+    par, ch, h, r, a = state
+    res = {}
+    for name, mode, seed in [("plain", "plain", 202), ("delayed", "delayed", 303)]:
+        c = Chain(wl["codes"], par, ch, h, r, wl["freqs"], a, seed=seed, **kw)
+        if mode == "delayed":
+            c.set_surrogate(sur)
+        trace, st, _ = c.run(args.mcmc_steps, mode=mode)
+        c.close()
+        e = {k: ess(trace[:, col]) for k, col in (("posterior", 1), ("tree_length", 5),
+                                                   ("alpha", 6))}
+        res[name] = dict(stats=st, ess=e, min_ess=min(e.values()),
+                         ess_per_s=min(e.values()) / st["seconds"])
+        log(f"[bench] {name}: {st['seconds']:.1f} s (likelihood {st['likelihood_seconds']:.1f} s, "
+            f"surrogate {st['surrogate_seconds']:.1f} s), accepted {st['op_accepted']} of "
+            f"{st['op_proposed']}, exact {st['exact_evals']}, ESS {e}")
+    ratio = res["delayed"]["ess_per_s"] / res["plain"]["ess_per_s"]
+    ratio_incl_training = (res["delayed"]["min_ess"] /
+                           (res["delayed"]["stats"]["seconds"] + t_train)) / res["plain"]["ess_per_s"]
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ncores = len(os.sched_getaffinity(0))
+        otree = O.Tree(L, wl["parents"], wl["children"], wl["heights"])
+        om = O.Model.gtr(wl["gtr_rates"], wl["freqs"])
+        tpc = min(ncores, (P + 999) // 1000)
+        lik = O.GammaLikelihood(otree, om, wl["codes"], P, wl["weights"], num_categories=4,
+                                alpha=wl["alpha"], threads_per_category=tpc,
+                                parallel_categories=True)
+        nev, t0 = 0, time.perf_counter()
+        while True:
+            otree.mark_all_edges_updated()
+            lik.likelihood()
+            lik.accept()
+            otree.accept()
+            nev += 1
+            if time.perf_counter() - t0 > args.cpu_seconds or nev >= 20:
+                break
+        dtc = time.perf_counter() - t0
+        # plain MH on the CPU would need the same exact evaluations per ESS
+        ps = res["plain"]["stats"]
+        evals_per_ess = ps["exact_evals"] / res["plain"]["min_ess"]
+        cpu = {"value": (nev / dtc) / evals_per_ess, "unit": "ESS/s (plain MH, upper bound)",
+               "cores": tpc * 4, "kind": "port",
+               "sample": f"{nev} full evaluations by the FP64 oracle ({dtc:.1f} s, "
+                         f"{nev / dtc:.2f} evals/s), divided by the plain GPU chain's "
+                         f"{evals_per_ess:.1f} exact evaluations per effective sample "
+                         f"(full evaluations only, so an upper bound for the CPU)"}
+    print(json.dumps({
+        "metric": METRIC, "value": res["delayed"]["ess_per_s"], "unit": "ESS/s",
+        "n_gpus": 1, "steps": args.mcmc_steps, "warmup": args.burnin,
+        "ms_per_step": res["delayed"]["stats"]["seconds"] / args.mcmc_steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Yule tree, sites simulated along it)",
+        "config": {"workload": wl["name"], "taxa": L, "patterns": P, "categories": 4,
+                   "model": "GTR+Gamma4 alpha 0.5", "surrogate": f"{nf}->256->256->1 GELU BF16",
+                   "burnin_steps": args.burnin, "chain_steps": args.mcmc_steps,
+                   "operators": "NodeSlide 4, TreeScale 1, DeltaExchange 2, ScaleReal(alpha) 1, "
+                                "NarrowExchange 2 (b3's defaults, tuned in burn-in)"},
+        "ess_per_s_plain": res["plain"]["ess_per_s"], "ess_per_s_delayed": res["delayed"]["ess_per_s"],
+        "ess_ratio_delayed_vs_plain": ratio,
+        "ess_ratio_including_training": ratio_incl_training,
+        "plain": res["plain"], "delayed": res["delayed"], "surrogate_held_out": held,
+        "burnin_stats": bstats, "cpu_baseline": cpu, "roofline": None,
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -263,7 +382,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-f32", action="store_true", help="profiling runs: FP32 kernel only")
-    ap.add_argument("--workload", choices=["gtr128", "batch", "codon"], default="gtr128")
+    ap.add_argument("--workload", choices=["gtr128", "batch", "codon", "mcmc"],
+                    default="gtr128")
+    ap.add_argument("--mcmc-taxa", type=int, default=200)
+    ap.add_argument("--mcmc-sites", type=int, default=200_000)
+    ap.add_argument("--mcmc-steps", type=int, default=100_000)
+    ap.add_argument("--burnin", type=int, default=30_000)
+    ap.add_argument("--epochs", type=int, default=100)
+    ap.add_argument("--clip", type=float, default=100.0)
     ap.add_argument("--codon-taxa", type=int, default=64)
     ap.add_argument("--codon-patterns", type=int, default=20_000)
     ap.add_argument("--instances", type=int, default=256)
@@ -278,6 +404,8 @@ def main():
         return bench_batch(args)
     if args.workload == "codon":
         return bench_codon(args)
+    if args.workload == "mcmc":
+        return bench_mcmc(args)
 
     from aspartik_amd import workloads
     from aspartik_amd.b3 import Calculator, DatedTree

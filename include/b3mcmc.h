@@ -83,8 +83,18 @@ typedef struct {
   double alpha_prior_rate;   /* Exponential prior on the Gamma shape */
   /* operator weights (mcmc.rs picks one operator per step in proportion) */
   double w_node_slide, w_tree_scale, w_rates, w_alpha, w_narrow;
-  double scale_factor;       /* ScaleReal / TreeScale factor f: scale ~ U(f, 1/f) */
+  double scale_factor;       /* TreeScale factor f: scale ~ U(f, 1/f) */
+  double alpha_factor;       /* ScaleReal factor on alpha (0: scale_factor) */
   double delta_tuning;       /* DeltaExchange tuning t: delta ~ U(0, (1 - t) / 10) */
+  /* Operator tuning as b3's Scheduler::tune with a LinearOptimizer
+   * (operators/mod.rs:164-190,467-482; optimizers.py:13-38): every
+   * tune_every steps each tunable operator's parameter moves by
+   * rate * (0.234 - acceptance since the last tune), clamped to [0.01, 0.99];
+   * rate = tune_magnitude until 0.9 tune_cutoff, then falls linearly to 0 at
+   * tune_cutoff (steps counted over the chain's life).  tune_every = 0: off. */
+  uint32_t tune_every;
+  uint64_t tune_cutoff;
+  double tune_magnitude;
 } b3m_chain_config;
 
 void b3m_chain_config_default(b3m_chain_config *cfg);
@@ -128,6 +138,7 @@ typedef struct {
   uint64_t op_proposed[5], op_accepted[5];  /* node slide, tree scale, rates, alpha, narrow */
   uint64_t collected;          /* (features, logL) pairs written */
   double seconds, likelihood_seconds, surrogate_seconds;
+  double tuning[5];            /* operator parameters at the end (-1: not tunable) */
 } b3m_run_stats;
 
 /* Runs `steps` steps.  Every `record_every` steps one trace row is written

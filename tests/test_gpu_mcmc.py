@@ -68,6 +68,21 @@ def test_surrogate_forward_and_gradients(hip_device, D, H, B):
         assert np.linalg.norm(got - ref) <= 5e-3 * np.linalg.norm(ref) + 1e-7
 
 
+def test_single_sample_path_matches_batched(hip_device):
+    """The fused one-sample forward used inside the chain equals the batched
+    kernels' result to FP32 rounding of the final reduction."""
+    s = Surrogate(404, 256, seed=17)
+    rng = np.random.default_rng(17)
+    p = s.params()
+    p[s.num_params - 257:] = rng.normal(size=257).astype(np.float32) * 0.3
+    s.set_params(p)
+    s.set_scaling(rng.normal(size=404), rng.uniform(0.5, 2.0, size=404), -1234.5, 17.0)
+    x = rng.normal(size=(9, 404)).astype(np.float32)
+    batched = s.forward(x)
+    single = np.array([s.forward(x[i:i + 1])[0] for i in range(9)])
+    np.testing.assert_allclose(single, batched, rtol=1e-6, atol=1e-4)
+
+
 def test_surrogate_gradient_is_a_derivative(hip_device):
     """Finite differences of the reference loss pin the reference gradient."""
     s = Surrogate(11, 16, seed=3)
