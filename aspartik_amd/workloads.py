@@ -156,7 +156,7 @@ def gy94_codon(num_leaves=64, num_patterns=20_000, omega=0.3, kappa=2.0, alpha=0
         if have >= num_patterns:
             break
     codes = np.ascontiguousarray(np.concatenate(parts, axis=1)[:, :num_patterns])
-    return dict(name=f"gy94_codon_{num_leaves}taxa_{num_patterns}pat", parents=parents,
+    return dict(name=f"gy94_{num_leaves}taxa_{num_patterns}pat", parents=parents,
                 children=children, heights=heights, model=model, codes=codes,
                 weights=np.ones(num_patterns, np.uint32), K=K, alpha=alpha, rates=cat,
                 seed=seed)

@@ -37,11 +37,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def read_traffic(workload):
+def read_traffic(workload, kernel="prune_f32"):
     """HBM bytes per launch of the pruning kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy
     kernel calibration), if one exists for this workload."""
-    path = os.path.join(ROOT, "profiles", f"prune_f32_{workload}_pmc.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -245,7 +245,8 @@ def bench_codon(args):
                    "precision": "FP16 MFMA inputs (P split hi+lo), FP32 accumulation, "
                                 "FP64 root/mix/sum", "seed": wl["seed"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic(wl["name"], "prune_codon")},
         "mfma": {"useful_tflops": tflops, "issued_tflops": tflops * 2 * (64 / 61) ** 2,
                  "peak_fp16_tflops": 2500.0},
         "cpu_baseline": cpu, "kernel_us": kernel_ms * 1e3, "algorithmic_bytes_per_eval": B,
