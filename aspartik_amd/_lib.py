@@ -131,6 +131,9 @@ SIGNATURES = {
     "b3h_propose": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, f64p, f64p]),
     "b3h_set_model": (C.c_int, [C.c_void_p, f64p, f64p, C.c_uint32]),
     "b3h_propose_times": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, f64p, f64p]),
+    "b3h_set_category_model": (C.c_int, [C.c_void_p, C.c_uint32, f64p, f64p, C.c_uint32]),
+    "b3h_propose_lengths": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, f64p]),
+    "b3h_set_classes": (C.c_int, [C.c_void_p, C.c_void_p]),
     "b3h_accept": (C.c_int, [C.c_void_p]),
     "b3h_reject": (C.c_int, [C.c_void_p]),
     "b3h_likelihoods": (C.c_int, [C.c_void_p, C.c_uint32, f64p]),

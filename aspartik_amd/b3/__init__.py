@@ -2,13 +2,16 @@
 tree, substitution and likelihood objects that drive the pruning path)."""
 from .batch import Batch
 from .calculator import Calculator, HipCalculator, gamma_rates
-from .likelihood import DNALikelihood, GammaLikelihood, deduplicate
+from .clock import Clock, RelaxedClock, StrictClock
+from .likelihood import DNALikelihood, GammaLikelihood, HeteroLikelihood, deduplicate
+from .parameters import ClassVector, Real
 from .mcmc import Chain, Surrogate, ess
 from .substitution import GTR, GY94, HKY, JC, K80, SENSE_CODONS
 from .tree import TOMBSTONE, DatedTree
 
 __all__ = [
     "Batch", "Calculator", "HipCalculator", "gamma_rates", "DNALikelihood", "GammaLikelihood",
+    "HeteroLikelihood", "Clock", "StrictClock", "RelaxedClock", "ClassVector", "Real",
     "deduplicate", "GTR", "GY94", "HKY", "JC", "K80", "SENSE_CODONS", "TOMBSTONE",
     "DatedTree", "Chain", "Surrogate", "ess",
 ]

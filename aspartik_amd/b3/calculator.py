@@ -113,6 +113,33 @@ class HipCalculator:
         check(lib().b3h_set_model(self._h, np.ascontiguousarray(q, np.float64).reshape(-1),
                                   np.ascontiguousarray(frequencies, np.float64), flags))
 
+    def set_category_model(self, category, q, frequencies, special_cases: bool = True):
+        """Model of one category alone (HeteroLikelihood, hetero.rs:122-135)."""
+        flags = _lib.B3H_MODEL_SPECIAL_CASES if special_cases else 0
+        check(lib().b3h_set_category_model(
+            self._h, int(category), np.ascontiguousarray(q, np.float64).reshape(-1),
+            np.ascontiguousarray(frequencies, np.float64), flags))
+
+    def propose_lengths(self, internals, children, lengths):
+        """lengths [K][2n]: branch length of each child edge per category."""
+        internals = np.ascontiguousarray(internals, np.uint32)
+        lengths = np.ascontiguousarray(lengths, np.float64).reshape(-1)
+        if lengths.size != self.K * 2 * internals.size:
+            raise ValueError("lengths must hold K x 2n values")
+        check(lib().b3h_propose_lengths(self._h, internals.size, internals,
+                                        np.ascontiguousarray(children, np.uint32).reshape(-1),
+                                        lengths))
+
+    def set_classes(self, classes):
+        """Per-pattern category (hetero.rs:150-160); None: category mean."""
+        if classes is None:
+            check(lib().b3h_set_classes(self._h, None))
+            return
+        classes = np.ascontiguousarray(classes, np.uint32)
+        if classes.size != self.P:
+            raise ValueError("classes must hold one value per pattern")
+        check(lib().b3h_set_classes(self._h, classes.ctypes.data_as(C.c_void_p)))
+
     def accept(self):
         check(lib().b3h_accept(self._h))
 

@@ -65,13 +65,14 @@ struct b3h_calc {
   int16_t *d_exps = nullptr;    // F32 path
   uint8_t *d_scales = nullptr;  // F64 path flags [2I][K][P]
   uint32_t *d_scale_sums = nullptr, *d_scale_sums_bak = nullptr;
-  double *d_site = nullptr, *d_mixed = nullptr;
+  double *d_site = nullptr, *d_mixed = nullptr;  // [2][K][P], [2][P]: result slots
   uint32_t *d_weights = nullptr;
   double *d_blocksum = nullptr, *d_mix_blocksum = nullptr;
   uint32_t *d_counter = nullptr;
-  double *d_total = nullptr;
+  double *d_total = nullptr;  // [2]
   void *d_pmats = nullptr;  // [I][2][K][S*S] float or double
-  DevModel *d_model = nullptr;
+  DevModel *d_model = nullptr;  // [K]; only [0] is used while one model serves all
+  uint8_t *d_classes = nullptr;  // [P] per-pattern category (HeteroLikelihood)
   double *d_ucodon = nullptr;  // MFMA path: [64][4096] eigenvector products
   bool model_set = false;
 
@@ -87,6 +88,16 @@ struct b3h_calc {
   std::vector<uint8_t> editbuf_next;
   Scheduler sched;
   double freqs[64] = {0};
+  // per-category models (hetero.rs:122-135): model[c] and its frequencies
+  bool per_category = false;
+  double freqs_k[16][4] = {{0}};
+  bool classes_on = false;
+  // Results (site values, mixed values, total) live in two slots, like the
+  // likelihood layer's SkSliceBuf (hetero.rs:30,149-156): a propose writes
+  // slot rslot ^ 1, accept makes it the accepted slot, reject keeps rslot.
+  uint32_t rslot = 0;
+  bool pending = false;   // a propose since the last accept / reject
+  bool accepted_valid = false;
   bool has_result = false;
   uint64_t bytes = 0;
 
@@ -116,7 +127,7 @@ b3h_status dev_alloc(b3h_calc *c, void **p, size_t bytes) {
 void release(b3h_calc *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void *ptrs[] = {c->d_codes, c->d_partials, c->d_exps, c->d_scales,
+  void *ptrs[] = {c->d_codes, c->d_partials, c->d_exps, c->d_scales, c->d_classes,
                   c->d_scale_sums, c->d_scale_sums_bak, c->d_site, c->d_mixed,
                   c->d_weights, c->d_blocksum, c->d_mix_blocksum,
                   c->d_counter, c->d_total, c->d_pmats, c->d_model, c->d_ucodon};
@@ -162,6 +173,14 @@ b3h_status check_handle(const b3h_calc *c) {
 
 b3h_status set_device(const b3h_calc *c) {
   HIPCHK(hipSetDevice(c->device));
+  return B3H_OK;
+}
+
+// Slot the result reads use: the pending propose's, else the accepted one.
+b3h_status result_slot(const b3h_calc *c, uint32_t *slot) {
+  if (c->pending) *slot = c->rslot ^ 1u;
+  else if (c->accepted_valid) *slot = c->rslot;
+  else return fail(B3H_ERR_STATE, c->has_result ? "no accepted result" : "no propose yet");
   return B3H_OK;
 }
 
@@ -218,12 +237,14 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     return fail(B3H_ERR_ARG, err);
   c->editbuf.swap(c->editbuf_next);
 
-  // stage: entries | matrices (mode 0) or times + rates (mode 1) | pi
-  // (the MFMA path stages host matrices already in its FP16 A layout)
+  // stage: entries | matrices (mode 0) or times + rates (mode 1) or branch
+  // lengths [K][2n] (mode 2) | pi (the MFMA path stages host matrices
+  // already in its FP16 A layout)
+  const int mode = mats ? 0 : rates ? 1 : 2;
   const size_t ebytes = (size_t)n * sizeof(Entry);
   const size_t mbytes = !mats ? 0 : codon ? (size_t)K * 2 * n * 8192 * 2 : (size_t)K * 2 * n * SS * 8;
-  const size_t tbytes = times ? (size_t)2 * n * 8 : 0;
-  const size_t rbytes = times ? (size_t)K * 8 : 0;
+  const size_t tbytes = !times ? 0 : (mode == 2 ? (size_t)K : 1u) * 2 * n * 8;
+  const size_t rbytes = rates ? (size_t)K * 8 : 0;
   const size_t off_m = (ebytes + 255) & ~(size_t)255;
   const size_t off_t = off_m + ((mbytes + 255) & ~(size_t)255);
   const size_t off_r = off_t + ((tbytes + 255) & ~(size_t)255);
@@ -257,12 +278,16 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
         }
     }
   }
-  if (times) {
-    std::memcpy(h + off_t, times, tbytes);
-    std::memcpy(h + off_r, rates, rbytes);
-  }
+  if (times) std::memcpy(h + off_t, times, tbytes);
+  if (rates) std::memcpy(h + off_r, rates, rbytes);
+  // root frequencies: [S], or [K][4] when the device models differ per
+  // category (K <= 16, so the 64-entry block holds them)
+  const bool pik = c->per_category && mode != 0;
   double *hpi = reinterpret_cast<double *>(h + off_pi);
   for (uint32_t j = 0; j < 64; j++) hpi[j] = j < S ? freqs[j] : 0.0;
+  if (pik)
+    for (uint32_t k = 0; k < K; k++)
+      for (uint32_t j = 0; j < 4; j++) hpi[4 * k + j] = c->freqs_k[k][j];
   unsigned char *d = c->d_stage[s];
   HIPCHK(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipEventRecord(c->stage_ev[s], c->stream));
@@ -271,8 +296,16 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
   const double *d_times = reinterpret_cast<const double *>(d + off_t);
   const double *d_rates = reinterpret_cast<const double *>(d + off_r);
 
-  for (uint32_t j = 0; j < S; j++) c->freqs[j] = freqs[j];
-  const int mode = mats ? 0 : 1;
+  if (mode == 0)
+    for (uint32_t j = 0; j < S; j++) c->freqs[j] = freqs[j];
+  const double *d_pik = pik ? reinterpret_cast<const double *>(d + off_pi) : nullptr;
+  const uint32_t mstride = c->per_category ? 1u : 0u;
+  const uint8_t *d_classes = c->classes_on ? c->d_classes : nullptr;
+  // results go to the slot that is not the accepted one
+  const uint32_t ws = c->rslot ^ 1u;
+  double *r_site = c->d_site + (size_t)ws * K * c->P;
+  double *r_mixed = c->d_mixed + (size_t)ws * c->P;
+  double *r_total = c->d_total + ws;
 
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->profiling) {
@@ -301,7 +334,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     a.codes = c->d_codes;
     a.partials = static_cast<_Float16 *>(c->d_partials);
     a.exps = c->d_exps;
-    a.site = c->d_site;
+    a.site = r_site;
     a.pi = reinterpret_cast<const double *>(d + off_pi);
     a.cstride = c->cstride;
     a.n = n;
@@ -313,19 +346,20 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     HIPCHK(launch_prune_codon(a, c->stream));
     if (e1) HIPCHK(hipEventRecord(e1, c->stream));
     MixArgs m{};
-    m.site = c->d_site;
-    m.mixed = c->d_mixed;
+    m.site = r_site;
+    m.classes = d_classes;
+    m.mixed = r_mixed;
     m.weights = c->d_weights;
     m.blocksum = c->d_mix_blocksum;
     m.counter = c->d_counter;
-    m.total = c->d_total;
+    m.total = r_total;
     m.ln_inv_k = std::log(1.0 / (double)K);
     m.P = c->P;
     m.K = K;
     HIPCHK(launch_mix(m, c->stream));
   } else if (f32) {
     HIPCHK(launch_pbuild<float>(d_entries, n, K, S, mode, d_mats, d_times,
-                                d_rates, c->d_model,
+                                d_rates, c->d_model, mstride,
                                 static_cast<float *>(c->d_pmats), c->stream));
     F32Args a{};
     a.entries = d_entries;
@@ -334,16 +368,18 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     a.cstride = c->cstride;
     a.partials = static_cast<float4 *>(c->d_partials);
     a.exps = c->d_exps;
-    a.site = c->d_site;
-    a.mixed = c->d_mixed;
+    a.site = r_site;
+    a.mixed = r_mixed;
     a.weights = c->d_weights;
     a.blocksum = c->d_blocksum;
     a.counter = c->d_counter;
-    a.total = c->d_total;
+    a.total = r_total;
     a.pi0 = freqs[0];
     a.pi1 = freqs[1];
     a.pi2 = freqs[2];
     a.pi3 = freqs[3];
+    a.pik = d_pik;
+    a.classes = d_classes;
     a.ln_inv_k = std::log(1.0 / (double)K);
     a.n = n;
     a.P = c->P;
@@ -363,7 +399,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     if (e1) HIPCHK(hipEventRecord(e1, c->stream));
   } else {
     HIPCHK(launch_pbuild<double>(d_entries, n, K, S, mode, d_mats, d_times,
-                                 d_rates, c->d_model,
+                                 d_rates, c->d_model, mstride,
                                  static_cast<double *>(c->d_pmats), c->stream));
     F64Args a{};
     a.entries = d_entries;
@@ -373,11 +409,12 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     a.partials = static_cast<double4 *>(c->d_partials);
     a.scales = c->d_scales;
     a.scale_sums = c->d_scale_sums;
-    a.site = c->d_site;
+    a.site = r_site;
     a.pi0 = freqs[0];
     a.pi1 = freqs[1];
     a.pi2 = freqs[2];
     a.pi3 = freqs[3];
+    a.pik = d_pik;
     a.scale_ln = c->scale_ln;
     a.mult = std::exp((double)c->scale_ln);
     a.threshold = std::exp(-(double)c->scale_ln);
@@ -388,12 +425,13 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     if (e0) HIPCHK(hipEventRecord(e0, c->stream));
     HIPCHK(launch_prune_f64(a, c->stream));
     MixArgs m{};
-    m.site = c->d_site;
-    m.mixed = c->d_mixed;
+    m.site = r_site;
+    m.classes = d_classes;
+    m.mixed = r_mixed;
     m.weights = c->d_weights;
     m.blocksum = c->d_mix_blocksum;
     m.counter = c->d_counter;
-    m.total = c->d_total;
+    m.total = r_total;
     m.ln_inv_k = std::log(1.0 / (double)K);
     m.P = c->P;
     m.K = K;
@@ -401,6 +439,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     if (e1) HIPCHK(hipEventRecord(e1, c->stream));
   }
   c->has_result = true;
+  c->pending = true;
   return B3H_OK;
 }
 
@@ -550,8 +589,9 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
     if ((st = dev_alloc(c, (void **)&c->d_scale_sums, (size_t)K * P * 4))) return bail(st);
     if ((st = dev_alloc(c, (void **)&c->d_scale_sums_bak, (size_t)K * P * 4))) return bail(st);
   }
-  if ((st = dev_alloc(c, (void **)&c->d_site, (size_t)K * P * 8))) return bail(st);
-  if ((st = dev_alloc(c, (void **)&c->d_mixed, (size_t)P * 8))) return bail(st);
+  if ((st = dev_alloc(c, (void **)&c->d_site, (size_t)2 * K * P * 8))) return bail(st);
+  if ((st = dev_alloc(c, (void **)&c->d_mixed, (size_t)2 * P * 8))) return bail(st);
+  if ((st = dev_alloc(c, (void **)&c->d_classes, (size_t)P))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_weights, (size_t)P * 4))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_blocksum, (size_t)std::max<uint32_t>(c->blocks, 1u) * 8)))
     return bail(st);
@@ -560,7 +600,7 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   if ((st = dev_alloc(c, (void **)&c->d_total, 64))) return bail(st);
   const size_t mat_bytes = codon ? 8192 * 2 : (size_t)S * S * (f32 ? 4 : 8);
   if ((st = dev_alloc(c, &c->d_pmats, (size_t)c->I * 2 * K * mat_bytes))) return bail(st);
-  if ((st = dev_alloc(c, (void **)&c->d_model, sizeof(DevModel)))) return bail(st);
+  if ((st = dev_alloc(c, (void **)&c->d_model, (size_t)K * sizeof(DevModel)))) return bail(st);
   if (codon && (st = dev_alloc(c, (void **)&c->d_ucodon, (size_t)64 * 4096 * 8))) return bail(st);
 
   // staging: entries + host matrices (the larger of the f64 input and the
@@ -568,7 +608,7 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   const size_t host_mat = std::max<size_t>((size_t)S * S * 8, codon ? 8192 * 2 : 0);
   c->stage_bytes = ((size_t)c->I * sizeof(Entry) + 255) / 256 * 256 +
                    ((size_t)K * 2 * c->I * host_mat + 255) / 256 * 256 +
-                   ((size_t)2 * c->I * 8 + 255) / 256 * 256 + ((size_t)K * 8 + 255) / 256 * 256 +
+                   ((size_t)K * 2 * c->I * 8 + 255) / 256 * 256 + ((size_t)K * 8 + 255) / 256 * 256 +
                    64 * 8 + 256;
   for (int s = 0; s < kStages; s++) {
     if (hipHostMalloc((void **)&c->h_stage[s], c->stage_bytes, hipHostMallocDefault) != hipSuccess) {
@@ -586,7 +626,8 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   bool ok = hipMemcpy(c->d_codes, codes.data(), codes.size(), hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(c->d_weights, ones.data(), (size_t)P * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemset(c->d_counter, 0, 64) == hipSuccess &&
-            hipMemset(c->d_model, 0, sizeof(DevModel)) == hipSuccess &&
+            hipMemset(c->d_model, 0, (size_t)K * sizeof(DevModel)) == hipSuccess &&
+            hipMemset(c->d_classes, 0, P) == hipSuccess &&
             hipMemset(c->d_partials, 0, punits * elem) == hipSuccess;
   if (ok && (f32 || codon)) ok = hipMemset(c->d_exps, 0, punits * 2) == hipSuccess;
   if (ok && !f32 && !codon)
@@ -655,6 +696,105 @@ b3h_status b3h_set_model(b3h_calc *c, const double *q, const double *freqs,
   HIPCHK(hipStreamSynchronize(c->stream));
   for (uint32_t i = 0; i < S; i++) c->freqs[i] = freqs[i];
   c->model_set = true;
+  c->per_category = false;
+  return B3H_OK;
+}
+
+b3h_status b3h_set_category_model(b3h_calc *c, uint32_t category, const double *q,
+                                  const double *freqs, uint32_t flags) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if (!q || !freqs) return fail(B3H_ERR_ARG, "null model arrays");
+  if (category >= c->K) return fail(B3H_ERR_ARG, "category out of range");
+  if (c->precision == B3H_F16_MFMA)
+    return fail(B3H_ERR_UNSUPPORTED, "per-category models need 4 states");
+  for (uint32_t i = 0; i < 4; i++)
+    if (!(freqs[i] > 0.0)) return fail(B3H_ERR_ARG, "frequencies must be positive");
+  if ((st = set_device(c))) return st;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!c->per_category) {
+    // leaving the shared model: every category starts from it (or from
+    // the new model where none was set yet)
+    for (uint32_t k = 0; k < c->K; k++) {
+      if (c->model_set && k != 0)
+        HIPCHK(hipMemcpyAsync(c->d_model + k, c->d_model, sizeof(DevModel),
+                              hipMemcpyDeviceToDevice, c->stream));
+      for (uint32_t j = 0; j < 4; j++) c->freqs_k[k][j] = c->model_set ? c->freqs[j] : freqs[j];
+    }
+  }
+  DevModel hm;
+  std::memset(&hm, 0, sizeof hm);
+  hm.n = 4;
+  hm.special = (flags & B3H_MODEL_SPECIAL_CASES) ? 1 : 0;
+  for (uint32_t i = 0; i < 4; i++) hm.freqs[i] = freqs[i];
+  for (uint32_t i = 0; i < 16; i++) hm.q[i] = q[i];
+  HIPCHK(hipMemcpyAsync(c->d_model + category, &hm, sizeof hm, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(launch_eigen(c->d_model + category, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (uint32_t j = 0; j < 4; j++) c->freqs_k[category][j] = freqs[j];
+  if (!c->model_set) {  // categories never given a model copy this one
+    for (uint32_t k = 0; k < c->K; k++)
+      if (k != category) {
+        HIPCHK(hipMemcpyAsync(c->d_model + k, c->d_model + category, sizeof(DevModel),
+                              hipMemcpyDeviceToDevice, c->stream));
+      }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (uint32_t j = 0; j < 4; j++) c->freqs[j] = freqs[j];
+  }
+  c->model_set = true;
+  c->per_category = true;
+  return B3H_OK;
+}
+
+b3h_status b3h_propose_lengths(b3h_calc *c, uint32_t n, const uint32_t *internals,
+                               const uint32_t *children, const double *lengths) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if (!c->model_set) return fail(B3H_ERR_STATE, "b3h_set_model must be called first");
+  if (!lengths) return fail(B3H_ERR_ARG, "null branch lengths");
+  if (c->precision == B3H_F16_MFMA)
+    return fail(B3H_ERR_UNSUPPORTED, "per-category branch lengths need 4 states");
+  double fr[64];
+  for (uint32_t j = 0; j < c->S; j++) fr[j] = c->freqs[j];
+  return run_propose(c, n, internals, children, nullptr, lengths, nullptr, fr);
+}
+
+b3h_status b3h_set_classes(b3h_calc *c, const uint32_t *classes) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if ((st = set_device(c))) return st;
+  if (!classes) {
+    c->classes_on = false;
+  } else {
+    std::vector<uint8_t> cls(c->P);
+    for (uint32_t i = 0; i < c->P; i++) {
+      if (classes[i] >= c->K)
+        return fail(B3H_ERR_ARG, "class out of range at pattern " + std::to_string(i));
+      cls[i] = (uint8_t)classes[i];
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->d_classes, cls.data(), c->P, hipMemcpyHostToDevice));
+    c->classes_on = true;
+  }
+  // recompute the total of the current result from its site values (the
+  // path of hetero.rs:150-160 when only the classes changed)
+  uint32_t slot;
+  if (c->pending) slot = c->rslot ^ 1u;
+  else if (c->accepted_valid) slot = c->rslot;
+  else return B3H_OK;
+  MixArgs m{};
+  m.site = c->d_site + (size_t)slot * c->K * c->P;
+  m.classes = c->classes_on ? c->d_classes : nullptr;
+  m.mixed = c->d_mixed + (size_t)slot * c->P;
+  m.weights = c->d_weights;
+  m.blocksum = c->d_mix_blocksum;
+  m.counter = c->d_counter;
+  m.total = c->d_total + slot;
+  m.ln_inv_k = std::log(1.0 / (double)c->K);
+  m.P = c->P;
+  m.K = c->K;
+  HIPCHK(launch_mix(m, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return B3H_OK;
 }
 
@@ -674,6 +814,11 @@ b3h_status b3h_accept(b3h_calc *c) {
   b3h_status st = check_handle(c);
   if (st) return st;
   for (auto &m : c->editbuf) m &= 1u;  // editbuf.rs:48-53
+  if (c->pending) {
+    c->rslot ^= 1u;
+    c->accepted_valid = true;
+    c->pending = false;
+  }
   if (c->precision == B3H_F64_PLAIN) {  // cpu4.rs:78-83
     if ((st = set_device(c))) return st;
     HIPCHK(hipMemcpyAsync(c->d_scale_sums_bak, c->d_scale_sums, (size_t)c->K * c->P * 4,
@@ -686,6 +831,7 @@ b3h_status b3h_reject(b3h_calc *c) {
   b3h_status st = check_handle(c);
   if (st) return st;
   for (auto &m : c->editbuf) m = (uint8_t)((m ^ (m >> 1)) & 1u);  // editbuf.rs:55-66
+  c->pending = false;
   if (c->precision == B3H_F64_PLAIN) {  // cpu4.rs:85-89
     if ((st = set_device(c))) return st;
     HIPCHK(hipMemcpyAsync(c->d_scale_sums, c->d_scale_sums_bak, (size_t)c->K * c->P * 4,
@@ -699,9 +845,10 @@ b3h_status b3h_likelihoods(b3h_calc *c, uint32_t category, double *out) {
   if (st) return st;
   if (!out) return fail(B3H_ERR_ARG, "null output");
   if (category >= c->K) return fail(B3H_ERR_ARG, "category out of range");
-  if (!c->has_result) return fail(B3H_ERR_STATE, "no propose yet");
+  uint32_t slot;
+  if ((st = result_slot(c, &slot))) return st;
   if ((st = set_device(c))) return st;
-  HIPCHK(hipMemcpyAsync(out, c->d_site + (size_t)category * c->P, (size_t)c->P * 8,
+  HIPCHK(hipMemcpyAsync(out, c->d_site + ((size_t)slot * c->K + category) * c->P, (size_t)c->P * 8,
                         hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return B3H_OK;
@@ -711,9 +858,11 @@ b3h_status b3h_site_mixed(b3h_calc *c, double *out) {
   b3h_status st = check_handle(c);
   if (st) return st;
   if (!out) return fail(B3H_ERR_ARG, "null output");
-  if (!c->has_result) return fail(B3H_ERR_STATE, "no propose yet");
+  uint32_t slot;
+  if ((st = result_slot(c, &slot))) return st;
   if ((st = set_device(c))) return st;
-  HIPCHK(hipMemcpyAsync(out, c->d_mixed, (size_t)c->P * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(out, c->d_mixed + (size_t)slot * c->P, (size_t)c->P * 8,
+                        hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return B3H_OK;
 }
@@ -722,9 +871,10 @@ b3h_status b3h_total(b3h_calc *c, double *out) {
   b3h_status st = check_handle(c);
   if (st) return st;
   if (!out) return fail(B3H_ERR_ARG, "null output");
-  if (!c->has_result) return fail(B3H_ERR_STATE, "no propose yet");
+  uint32_t slot;
+  if ((st = result_slot(c, &slot))) return st;
   if ((st = set_device(c))) return st;
-  HIPCHK(hipMemcpyAsync(out, c->d_total, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(out, c->d_total + slot, 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return B3H_OK;
 }

@@ -7,11 +7,17 @@ namespace b3h {
 
 // ---- model.hip ----------------------------------------------------------
 hipError_t launch_eigen(DevModel *m, hipStream_t s);
+// Transition matrices of one propose, [entry][side][category][S*S]:
+//   mode 0: cast of host matrices mats[K][2n][S*S]
+//   mode 1: P(times[m] * rates[c]) from the device eigensystem
+//   mode 2: P(times[c * 2n + m]), branch lengths given per category
+// model_stride 0: one model for every category; 1: model[c] per category.
 template <typename T>
 hipError_t launch_pbuild(const Entry *entries, uint32_t n, uint32_t K,
                          uint32_t S, int mode, const double *mats,
                          const double *times, const double *rates,
-                         const DevModel *model, T *out, hipStream_t s);
+                         const DevModel *model, uint32_t model_stride, T *out,
+                         hipStream_t s);
 
 struct PbuildDesc {
   const Entry *entries;
@@ -42,6 +48,8 @@ struct F32Args {
   uint32_t *counter;      // last-block ticket (self-resetting)
   double *total;
   double pi0, pi1, pi2, pi3;
+  const double *pik;        // [K][4] root frequencies per category, or null (pi0..pi3)
+  const uint8_t *classes;   // [P] category selected per pattern, or null (mean over categories)
   double ln_inv_k;
   uint32_t n, P, K, T, nslots;
   uint32_t cstride;       // code row stride (a multiple of 64 T)
@@ -94,6 +102,7 @@ struct F64Args {
   uint32_t *scale_sums;   // [K][P]
   double *site;           // [K][P]
   double pi0, pi1, pi2, pi3;
+  const double *pik;      // [K][4] per category, or null (pi0..pi3)
   double threshold, mult;
   uint32_t scale_ln;
   uint32_t n, P, K, L;
@@ -102,8 +111,11 @@ struct F64Args {
 hipError_t launch_prune_f64(const F64Args &a, hipStream_t s);
 
 // Category mix + weighted sum with a deterministic last-block reduction.
+// With classes, pattern i takes category classes[i]'s value instead of the
+// mean over categories.
 struct MixArgs {
   const double *site;  // [K][P]
+  const uint8_t *classes;  // [P] or null
   double *mixed;       // [P]
   const uint32_t *weights;
   double *blocksum;

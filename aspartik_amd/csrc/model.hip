@@ -105,12 +105,16 @@ __device__ inline double p_element(const DevModel *m, int S, double t, int r,
 // Build the transition matrices of one propose.  Output layout
 // [entry][side][category][S*S] in the kernels' entry order; `meta >> 8` maps
 // an entry to the caller's list position.  Mode 0 casts host matrices given
-// as [K][2n][S*S] f64; mode 1 builds them from edge_times[2n] x rates[K].
+// as [K][2n][S*S] f64; mode 1 builds them from edge_times[2n] x rates[K]
+// (gamma.rs:126-134: time length x (clock rate x category rate)); mode 2
+// from branch lengths [K][2n] formed by the caller, as transitions.rs:33-35
+// forms them for a per-edge (relaxed) clock.  With model_stride 1 category c
+// uses model[c] (hetero.rs:122-135: one substitution model per category).
 template <typename T>
 __global__ void pbuild_kernel(const Entry *entries, uint32_t n, uint32_t K,
                               uint32_t S, int mode, const double *mats,
                               const double *times, const double *rates,
-                              const DevModel *model, T *out) {
+                              const DevModel *model, uint32_t model_stride, T *out) {
   const uint32_t SS = S * S;
   const uint64_t total = (uint64_t)n * 2 * K * SS;
   for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
@@ -126,8 +130,9 @@ __global__ void pbuild_kernel(const Entry *entries, uint32_t n, uint32_t K,
     if (mode == 0) {
       v = mats[((uint64_t)c * 2 * n + 2ull * orig + side) * SS + el];
     } else {
-      const double t = times[2 * orig + side] * rates[c];
-      v = p_element(model, (int)S, t, (int)(el / S), (int)(el % S));
+      const double t = mode == 1 ? times[2 * orig + side] * rates[c]
+                                 : times[(uint64_t)c * 2 * n + 2 * orig + side];
+      v = p_element(model + c * model_stride, (int)S, t, (int)(el / S), (int)(el % S));
     }
     out[g] = (T)v;
   }
@@ -173,25 +178,26 @@ template <typename T>
 hipError_t launch_pbuild(const Entry *entries, uint32_t n, uint32_t K,
                          uint32_t S, int mode, const double *mats,
                          const double *times, const double *rates,
-                         const DevModel *model, T *out, hipStream_t s) {
+                         const DevModel *model, uint32_t model_stride, T *out,
+                         hipStream_t s) {
   const uint64_t total = (uint64_t)n * 2 * K * S * S;
   uint64_t blocks = (total + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(pbuild_kernel<T>, dim3((uint32_t)blocks), dim3(256), 0, s,
-                     entries, n, K, S, mode, mats, times, rates, model, out);
+                     entries, n, K, S, mode, mats, times, rates, model, model_stride, out);
   return hipGetLastError();
 }
 
 template hipError_t launch_pbuild<float>(const Entry *, uint32_t, uint32_t,
                                          uint32_t, int, const double *,
                                          const double *, const double *,
-                                         const DevModel *, float *,
+                                         const DevModel *, uint32_t, float *,
                                          hipStream_t);
 template hipError_t launch_pbuild<double>(const Entry *, uint32_t, uint32_t,
                                           uint32_t, int, const double *,
                                           const double *, const double *,
-                                          const DevModel *, double *,
+                                          const DevModel *, uint32_t, double *,
                                           hipStream_t);
 
 }  // namespace b3h

@@ -377,15 +377,24 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
   }
 
   // root: ln(pi . x) + E ln 2 in FP64 (cpu4.rs:277-284, cpu4.rs:69-73); the
-  // last node-loop barrier separates the stacks from the mix buffers
+  // last node-loop barrier separates the stacks from the mix buffers.  With
+  // per-category models each category has its own frequencies (hetero.rs:
+  // 122-145 proposes each calculator with its own transitions).
+  double pi0 = a.pi0, pi1 = a.pi1, pi2 = a.pi2, pi3 = a.pi3;
+  if (a.pik) {
+    pi0 = a.pik[4 * c];
+    pi1 = a.pik[4 * c + 1];
+    pi2 = a.pik[4 * c + 2];
+    pi3 = a.pik[4 * c + 3];
+  }
 #pragma unroll
   for (int q = 0; q < PPL; q++) {
     double site = -INFINITY;
     if (active[q]) {
-      double v = (double)x[q].x * a.pi0;
-      v = fma((double)x[q].y, a.pi1, v);
-      v = fma((double)x[q].z, a.pi2, v);
-      v = fma((double)x[q].w, a.pi3, v);
+      double v = (double)x[q].x * pi0;
+      v = fma((double)x[q].y, pi1, v);
+      v = fma((double)x[q].z, pi2, v);
+      v = fma((double)x[q].w, pi3, v);
       site = fma((double)E[q], LN2, log(v));
       a.site[(size_t)c * a.P + p[q]] = site;
     }
@@ -393,7 +402,21 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
   }
   __syncthreads();
   double wv = 0.0;
-  if (c == 0) {
+  if (c == 0 && a.classes) {
+    // HeteroLikelihood: the pattern's class picks one category's value
+    // (hetero.rs:150-160)
+#pragma unroll
+    for (int q = 0; q < PPL; q++) {
+      if (active[q]) {
+        const uint32_t k = a.classes[p[q]];
+        const double mixed = mix[((t * K + k) * PPL + q) * 64 + lane];
+        a.mixed[p[q]] = mixed;
+        wv = fma(mixed, (double)a.weights[p[q]], wv);
+      }
+    }
+    wv = wave_sum(wv);
+    if (lane == 0) red[t] = wv;
+  } else if (c == 0) {
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
       double mxv = -INFINITY;

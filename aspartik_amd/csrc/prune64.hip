@@ -89,7 +89,9 @@ __global__ __launch_bounds__(256) void prune_f64_kernel(F64Args a) {
     a.partials[onew] = make_double4(p0, p1, p2, p3);
     x0 = p0; x1 = p1; x2 = p2; x3 = p3;
   }
-  const double h0 = x0 * a.pi0, h1 = x1 * a.pi1, h2 = x2 * a.pi2, h3 = x3 * a.pi3;
+  const double *pi = a.pik ? a.pik + 4 * c : nullptr;
+  const double h0 = x0 * (pi ? pi[0] : a.pi0), h1 = x1 * (pi ? pi[1] : a.pi1),
+               h2 = x2 * (pi ? pi[2] : a.pi2), h3 = x3 * (pi ? pi[3] : a.pi3);
   double lik = log(((h0 + h1) + h2) + h3);
   lik -= (double)(uint32_t)(ss * a.scale_ln);
   a.site[cp] = lik;
@@ -109,13 +111,17 @@ __global__ __launch_bounds__(256) void mix_kernel(MixArgs a) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   double wv = 0.0;
   if (p < a.P) {
-    double mx = -INFINITY;
-    for (uint32_t k = 0; k < a.K; k++) mx = fmax(mx, a.site[(size_t)k * a.P + p]);
     double mixed = -INFINITY;
-    if (mx != -INFINITY) {
-      double se = 0.0;
-      for (uint32_t k = 0; k < a.K; k++) se += exp(a.site[(size_t)k * a.P + p] - mx);
-      mixed = mx + log(se) + a.ln_inv_k;
+    if (a.classes) {  // hetero.rs:150-160
+      mixed = a.site[(size_t)a.classes[p] * a.P + p];
+    } else {
+      double mx = -INFINITY;
+      for (uint32_t k = 0; k < a.K; k++) mx = fmax(mx, a.site[(size_t)k * a.P + p]);
+      if (mx != -INFINITY) {
+        double se = 0.0;
+        for (uint32_t k = 0; k < a.K; k++) se += exp(a.site[(size_t)k * a.P + p] - mx);
+        mixed = mx + log(se) + a.ln_inv_k;
+      }
     }
     a.mixed[p] = mixed;
     wv = mixed * (double)a.weights[p];

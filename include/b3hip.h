@@ -138,12 +138,60 @@ b3h_status b3h_propose_times(b3h_calc *calc, uint32_t n,
                              const double *edge_times,
                              const double *category_rates);
 
-/* Calculator::accept / reject (calculator/mod.rs:62-69; cpu4.rs:78-89). */
+/*
+ * Per-category models and branch lengths: HeteroLikelihood and relaxed
+ * clocks on one handle.
+ *
+ * b3's HeteroLikelihood (likelihood/hetero.rs:97-212) keeps one calculator,
+ * one substitution model and one clock per category, and each pattern's
+ * class (a ClassVector) picks which category's ln L enters the total.  Here
+ * the K categories of one handle play those calculators, so one launch
+ * evaluates them all and the class-select sum runs on the device.
+ *
+ * b3h_set_category_model: the device eigensystem and root frequencies of
+ *   category k alone (4 states; FP32 and FP64 paths).  The first call
+ *   gives every other category the current shared model (or this one, if
+ *   none was set); b3h_set_model returns to one model for all categories.
+ *   b3h_propose (host matrices) still takes one frequency vector.
+ */
+b3h_status b3h_set_category_model(b3h_calc *calc, uint32_t category,
+                                  const double *q, const double *frequencies,
+                                  uint32_t flags);
+
+/*
+ * propose with P built on the GPU from branch lengths the caller formed:
+ *   lengths[K][2n]: category-major, children order within a category; the
+ *   length of child m in category c as Transitions::update forms it
+ *   (transitions.rs:33-35): time length x the clock's rate for that edge,
+ *   which for a relaxed clock is the rate of the edge's rate category
+ *   (clock.rs:115-119), times the category rate for GammaLikelihood
+ *   (gamma.rs:131-133).
+ */
+b3h_status b3h_propose_lengths(b3h_calc *calc, uint32_t n,
+                               const uint32_t *internals,
+                               const uint32_t *children,
+                               const double *lengths);
+
+/*
+ * Per-pattern classes (hetero.rs:150-160): with classes set, the total is
+ * sum_i w_i lnL_{classes[i]}(i) and b3h_site_mixed returns those values;
+ * null returns to the category mean (GammaLikelihood).  classes[P], each
+ * < K.  If a result exists (pending, else accepted) its mixed values and
+ * total are recomputed from its site values without a propose: the
+ * hetero.rs:117-120,150-160 path when only the classes changed.
+ */
+b3h_status b3h_set_classes(b3h_calc *calc, const uint32_t *classes);
+
+/* Calculator::accept / reject (calculator/mod.rs:62-69; cpu4.rs:78-89).
+ * Results are kept in two slots like the likelihood layer's buffer
+ * (hetero.rs:30,170-212): after accept the proposal's results are current;
+ * after reject the previously accepted results are. */
 b3h_status b3h_accept(b3h_calc *calc);
 b3h_status b3h_reject(b3h_calc *calc);
 
 /* Calculator::likelihoods (calculator/mod.rs:71-72): per-pattern ln L of
- * category k, net of scaling.  Blocks until the last propose finishes. */
+ * category k, net of scaling, of the pending propose (else of the accepted
+ * state).  Blocks until the last propose finishes. */
 b3h_status b3h_likelihoods(b3h_calc *calc, uint32_t category, double *out);
 
 /* Per-pattern mixed value LSE_c(lnL_c) + ln(1/K) (gamma.rs:159-163); for

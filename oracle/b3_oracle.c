@@ -1157,6 +1157,62 @@ const double *or_gamma_lik_site(const or_gamma_lik *g, int c) {
 const double *or_gamma_lik_rates(const or_gamma_lik *g) { return g->rates; }
 
 /* ======================================================================== */
+/* Per-category models with per-edge clock rates, evaluated from scratch.
+ * The likelihood layers' totals over K calculators:
+ *   HeteroLikelihood (hetero.rs:122-160): calculator c proposes with its own
+ *     transitions (model c, rate of clock c per edge); pattern i adds
+ *     w_i * lnL_{classes[i]}(i), summed serially in pattern order.
+ *   GammaLikelihood (gamma.rs:126-166) when classes is NULL: LSE over the
+ *     categories plus ln(1/K), weighted (or_gamma_mix); K == 1 is
+ *     DnaLikelihood's weighted sum (dna.rs:96-97).
+ * Branch length of edge e in category c: edge_length(e) * edge_rates[c][e]
+ * (transitions.rs:33-35, with clock.get_rate(edge) [* category rate]
+ * already formed by the caller, clock.rs:115-119 / gamma.rs:131-133).
+ * Every internal node is recomputed (all edges marked), so the value does
+ * not depend on any earlier state. */
+double or_components_eval(const or_tree *t, int K, const or_model *models,
+                          const double *edge_rates, const uint8_t *codes,
+                          size_t P, const uint32_t *weights,
+                          const uint32_t *classes, uint32_t scale_ln,
+                          double *site) {
+  if (K < 1 || K > 64) return NAN;
+  uint8_t *all = malloc(t->N);
+  memset(all, 1, t->N);
+  uint32_t *internals = malloc(t->N * 4), *children = malloc(2 * (size_t)t->N * 4);
+  uint32_t n = updated_internals(t, all, internals);
+  children_list(t, internals, n, children);
+  double *mats = malloc((size_t)2 * n * 16 * 8);
+  double *sites = malloc((size_t)K * P * 8);
+  for (int c = 0; c < K; c++) {
+    for (uint32_t m = 0; m < 2 * n; m++) {
+      uint32_t e = children[m];
+      double len = or_tree_edge_length(t, e) * edge_rates[(size_t)c * t->N + e];
+      or_model_transition(&models[c], len, mats + (size_t)m * 16);
+    }
+    double freqs[4];
+    or_model_frequencies(&models[c], freqs);
+    or_calc4 *calc = or_calc4_new(P, codes, t->L, scale_ln, 1);
+    or_calc4_propose(calc, n, internals, children, mats, freqs);
+    memcpy(sites + (size_t)c * P, or_calc4_likelihoods(calc), P * 8);
+    or_calc4_free(calc);
+  }
+  double total = 0.0;
+  if (classes) {
+    for (size_t i = 0; i < P; i++)
+      total += sites[(size_t)classes[i] * P + i] * (double)weights[i];
+  } else if (K == 1) {
+    total = or_weighted_sum(sites, weights, P);
+  } else {
+    const double *rows[64];
+    for (int c = 0; c < K; c++) rows[c] = sites + (size_t)c * P;
+    total = or_gamma_mix(K, P, rows, weights, NULL);
+  }
+  if (site) memcpy(site, sites, (size_t)K * P * 8);
+  free(all); free(internals); free(children); free(mats); free(sites);
+  return total;
+}
+
+/* ======================================================================== */
 /* deduplicate (likelihood/mod.rs:74-132).  Columns are compared by content
  * (the reference compares blake3 digests of the column bytes); patterns are
  * numbered in order of first occurrence. */

@@ -145,6 +145,17 @@ void or_gamma_lik_set_alpha(or_gamma_lik *g, double alpha);
 const double *or_gamma_lik_site(const or_gamma_lik *g, int category);
 const double *or_gamma_lik_rates(const or_gamma_lik *g);
 
+/* ---- Per-category models and per-edge clock rates, from scratch -------- */
+/* HeteroLikelihood's total (hetero.rs:122-160) when classes != NULL, else
+ * GammaLikelihood's mix (gamma.rs:126-166).  models[K]; edge_rates[K][N]
+ * (N = 2L - 1 nodes, edge e = node e); codes [L x P]; writes the K x P site
+ * log likelihoods to site when non-NULL. */
+double or_components_eval(const or_tree *tree, int num_categories,
+                          const or_model *models, const double *edge_rates,
+                          const uint8_t *codes, size_t num_patterns,
+                          const uint32_t *weights, const uint32_t *classes,
+                          uint32_t scale_ln, double *site);
+
 /* ---- Pattern deduplication (crates/b3/src/likelihood/mod.rs:74-132) ----- */
 /* msa: [num_seqs x num_sites] byte codes (seq-major).  Returns P; writes
  * leaves[num_seqs x P], weights[P], sites_to_patterns[num_sites]. */

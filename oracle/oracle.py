@@ -105,6 +105,8 @@ def lib():
         "or_gamma_lik_site": (C.POINTER(C.c_double), [vp, C.c_int]),
         "or_gamma_lik_rates": (C.POINTER(C.c_double), [vp]),
         "or_deduplicate": (C.c_size_t, [C.c_uint32, C.c_size_t, u8p, u8p, u32p, szp]),
+        "or_components_eval": (C.c_double, [vp, C.c_int, vp, f64p, u8p, C.c_size_t, u32p, vp,
+                                            C.c_uint32, f64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -424,6 +426,28 @@ class GammaLikelihood:
     def rates(self):
         p = lib().or_gamma_lik_rates(self._h)
         return np.ctypeslib.as_array(p, shape=(self.K,)).copy()
+
+
+def components_eval(tree: Tree, models, edge_rates, codes, weights, classes=None, scale_ln=30):
+    """Per-category models and per-edge clock rates from scratch: HeteroLikelihood's
+    class-select total (crates/b3/src/likelihood/hetero.rs:122-160) when
+    ``classes`` is given, else GammaLikelihood's mix (gamma.rs:126-166).
+    edge_rates: [K][2L-1] (clock rate of each edge, times the category rate for
+    the gamma mix).  Returns (total, site [K][P])."""
+    K = len(models)
+    arr = (_Model * K)()
+    for c, m in enumerate(models):
+        arr[c] = m._m
+    codes = np.ascontiguousarray(codes, np.uint8)
+    P = codes.shape[1]
+    er = np.ascontiguousarray(edge_rates, np.float64).reshape(K, tree.N)
+    cls = None if classes is None else np.ascontiguousarray(classes, np.uint32)
+    site = np.zeros(K * P, np.float64)
+    tot = lib().or_components_eval(tree._h, K, C.cast(arr, C.c_void_p), er.reshape(-1),
+                                   codes.reshape(-1), P, np.ascontiguousarray(weights, np.uint32),
+                                   None if cls is None else cls.ctypes.data_as(C.c_void_p),
+                                   int(scale_ln), site)
+    return tot, site.reshape(K, P)
 
 
 def deduplicate(msa):

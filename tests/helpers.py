@@ -55,9 +55,10 @@ def tip_vector(code, S=4):
     return v
 
 
-def brute_force_site_ll(parents, heights, codes_col, transition, freqs, S=4):
+def brute_force_site_ll(parents, heights, codes_col, transition, freqs, S=4, edge_rates=None):
     """ln sum over all assignments of internal states of
     pi(root) prod_edges P(t)[parent, child]; tips fixed (unknown: summed).
+    With edge_rates, edge n's length is its time length times edge_rates[n].
     Pure-Python enumeration, only for tiny trees."""
     N = parents.size
     L = (N + 1) // 2
@@ -67,7 +68,8 @@ def brute_force_site_ll(parents, heights, codes_col, transition, freqs, S=4):
     for n in range(N):
         if n == root:
             continue
-        Pm[n] = transition(heights[parents[n]] - heights[n])
+        t = heights[parents[n]] - heights[n]
+        Pm[n] = transition(t if edge_rates is None else t * edge_rates[n])
     tipv = [tip_vector(int(codes_col[i]), S) for i in range(L)]
     total = 0.0
     for assign in itertools.product(range(S), repeat=len(internals)):
