@@ -15,6 +15,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload batch [--instances 256]   (batched HKY+Gamma4)
        python bench.py --workload codon                     (GY94 61-state, MFMA)
        python bench.py --workload mcmc                      (ESS/s, delayed acceptance)
+       python bench.py --workload hetero                    (HeteroLikelihood, relaxed clocks)
 """
 from __future__ import annotations
 
@@ -373,6 +374,115 @@ def bench_mcmc(args):
     }), flush=True)
 
 
+def bench_hetero(args):
+    """HeteroLikelihood at the GTR+Gamma4 workload's size: K = 4 categories,
+    each with its own GTR model and a relaxed LogNormal clock over 8 rate
+    categories, a random class per pattern.  One step is one full
+    evaluation (branch lengths uploaded, P built per category model, one
+    pruning launch for all categories, class-select weighted sum).  Also
+    timed: the classes-only path (total recomputed from the site values)."""
+    from aspartik_amd import distributions as D
+    from aspartik_amd import workloads
+    from aspartik_amd.b3 import GTR, Calculator, ClassVector, Clock, DatedTree
+
+    log("[bench] building workload")
+    wl = workloads.gtr_gamma4(args.taxa, args.patterns)
+    L, P, K = args.taxa, args.patterns, 4
+    N = 2 * L - 1
+    tree = DatedTree(L, wl["parents"], wl["children"], wl["heights"])
+    full = workloads.full_lists(tree)
+    rng = np.random.default_rng(2024)
+    subs = [GTR(rng.dirichlet([20.0] * 4), rng.dirichlet([4.0] * 6)) for _ in range(K)]
+    clocks = [Clock.Relaxed(ClassVector(8, N, rng.integers(0, 8, size=N)), D.LogNormal(0.0, 0.3))
+              for _ in range(K)]
+    classes = rng.integers(0, K, size=P).astype(np.uint32)
+    times = workloads.edge_times(tree, full[1])
+    ch = full[1].reshape(-1).astype(np.int64)
+    lengths = np.stack([times * c.edge_rates(N)[ch] for c in clocks])
+    name = f"hetero4_relaxed_{L}taxa_{P}pat"
+
+    calc = Calculator.HIP().make(wl["codes"], P, K)
+    calc.set_weights(wl["weights"])
+    for c, s in enumerate(subs):
+        calc.set_category_model(c, s.q_matrix(), s.frequencies())
+    calc.set_classes(classes)
+    for _ in range(args.warmup):
+        calc.propose_lengths(*full, lengths)
+        calc.total()
+        calc.accept()
+    calc.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        calc.propose_lengths(*full, lengths)
+        calc.total()
+        calc.accept()
+    dt = time.perf_counter() - t0
+    kms, nl = calc.profile_read()
+    calc.profile_enable(False)
+    kernel_ms = kms / max(1, nl)
+    total = calc.total()
+    log(f"[bench] hetero full: {args.steps / dt:.1f} evals/s, kernel {kernel_ms * 1e3:.1f} us")
+    # classes-only changes: ClassvecFlip-style edits of 1% of the patterns
+    perms = []
+    for i in range(8):
+        cl = classes.copy()
+        idx = rng.integers(0, P, size=P // 100)
+        cl[idx] = rng.integers(0, K, size=idx.size)
+        perms.append(cl)
+    t0 = time.perf_counter()
+    ncls = max(50, args.steps)
+    for i in range(ncls):
+        calc.set_classes(perms[i % len(perms)])
+        calc.total()
+    dt_cls = time.perf_counter() - t0
+    calc.set_classes(classes)
+    total = calc.total()
+    calc.close()
+
+    B = workloads.algorithmic_bytes(full[0], full[1], L, tree.root, K, P)
+    achieved = B / (kernel_ms * 1e-3) / 1e9
+    cpu = err = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        log("[bench] FP64 CPU oracle (one from-scratch evaluation)")
+        otree = O.Tree(L, wl["parents"], wl["children"], wl["heights"])
+        models = [O.Model.gtr(s.rates, s.freqs) for s in subs]
+        er = np.stack([c.edge_rates(N) for c in clocks])
+        t0 = time.perf_counter()
+        ref, _ = O.components_eval(otree, models, er, wl["codes"], wl["weights"], classes=classes)
+        dtc = time.perf_counter() - t0
+        cpu = {"value": 1.0 / dtc, "unit": "logL evals/s", "cores": 1, "kind": "port",
+               "sample": f"1 full evaluation of the same {L}-taxon {P}-pattern 4-category "
+                         f"workload, one thread ({dtc:.1f} s)"}
+        err = {"total_rel": abs(total - ref) / abs(ref), "ref_total": ref, "gpu_total": total}
+        log(f"[bench] oracle {dtc:.2f} s/eval; rel err {err['total_rel']:.2e}")
+    line = {
+        "metric": "HeteroLikelihood logL evals/s (relaxed clocks, per-category models)",
+        "value": args.steps / dt,
+        "unit": "logL evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Yule tree, sequences simulated along it; random classes "
+                "and clock categories)",
+        "config": {"workload": name, "taxa": L, "patterns": P, "categories": K,
+                   "model": "4 x GTR, relaxed LogNormal(0, 0.3) clocks, 8 rate categories",
+                   "evaluation": "full (all edges dirty)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(name)},
+        "cpu_baseline": cpu,
+        "kernel_us": kernel_ms * 1e3,
+        "classes_only_updates_per_s": ncls / dt_cls,
+        "error_vs_fp64_oracle": err,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,7 +493,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-f32", action="store_true", help="profiling runs: FP32 kernel only")
-    ap.add_argument("--workload", choices=["gtr128", "batch", "codon", "mcmc"],
+    ap.add_argument("--workload", choices=["gtr128", "batch", "codon", "mcmc", "hetero"],
                     default="gtr128")
     ap.add_argument("--mcmc-taxa", type=int, default=200)
     ap.add_argument("--mcmc-sites", type=int, default=200_000)
@@ -407,6 +517,8 @@ def main():
         return bench_codon(args)
     if args.workload == "mcmc":
         return bench_mcmc(args)
+    if args.workload == "hetero":
+        return bench_hetero(args)
 
     from aspartik_amd import workloads
     from aspartik_amd.b3 import Calculator, DatedTree
