@@ -30,6 +30,8 @@ STATUS_NAMES = {
 B3H_F32_SCALED = 0
 B3H_F64_PLAIN = 1
 B3H_F16_MFMA = 2
+B3H_F64_PROPAGATION = 3
+B3H_F32_PROPAGATION = 4
 B3H_MODEL_SPECIAL_CASES = 1
 
 
@@ -131,6 +133,10 @@ SIGNATURES = {
     "b3h_propose": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, f64p, f64p]),
     "b3h_set_model": (C.c_int, [C.c_void_p, f64p, f64p, C.c_uint32]),
     "b3h_propose_times": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, f64p, f64p]),
+    "b3h_propose_propagations": (C.c_int, [C.c_void_p, C.c_uint32, u32p, C.c_uint32, u32p,
+                                           f64p, f64p]),
+    "b3h_propose_propagations_times": (C.c_int, [C.c_void_p, C.c_uint32, u32p, C.c_uint32, u32p,
+                                                 f64p, f64p]),
     "b3h_set_category_model": (C.c_int, [C.c_void_p, C.c_uint32, f64p, f64p, C.c_uint32]),
     "b3h_propose_lengths": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, f64p]),
     "b3h_set_classes": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -44,6 +44,22 @@ class Calculator:
         (GY94 codon models: 61)."""
         return cls(_lib.B3H_F16_MFMA, device, states=states)
 
+    @classmethod
+    def HIPPropagationsF64(cls, device: int = 0):
+        """Propagation storage at FP64: b3's default GPU calculator
+        (calculator/cuda_propagations, mod.rs:114-128 ``CUDAPropagations``)."""
+        return cls(_lib.B3H_F64_PROPAGATION, device)
+
+    @classmethod
+    def HIPPropagations(cls, device: int = 0):
+        """Propagation storage with FP32 projections and power-of-two
+        exponents: the storage alternative to ``HIP()``."""
+        return cls(_lib.B3H_F32_PROPAGATION, device)
+
+    @property
+    def propagations(self) -> bool:
+        return self.precision in (_lib.B3H_F64_PROPAGATION, _lib.B3H_F32_PROPAGATION)
+
     def with_scale(self, scale_ln: int):
         """calculator/mod.rs:149-155"""
         self.scale_ln = int(scale_ln)
@@ -55,7 +71,8 @@ class Calculator:
         return HipCalculator(self, samples, num_patterns, num_categories)
 
     def __repr__(self):
-        names = {0: "HIP", 1: "HIPPlainF64", 2: "HIPCodon"}
+        names = {0: "HIP", 1: "HIPPlainF64", 2: "HIPCodon", 3: "HIPPropagationsF64",
+                 4: "HIPPropagations"}
         return (f"Calculator.{names.get(self.precision, self.precision)}(device={self.device}, "
                 f"scale_ln={self.scale_ln})")
 
@@ -81,6 +98,7 @@ class HipCalculator:
         h = C.c_void_p()
         check(lib().b3h_create(C.byref(cfg), samples, self.L, self.P, self.K, C.byref(h)))
         self._h = h
+        self.propagations = config.propagations
 
     def close(self):
         if getattr(self, "_h", None):
@@ -100,6 +118,26 @@ class HipCalculator:
                                 np.ascontiguousarray(children, np.uint32).reshape(-1),
                                 np.ascontiguousarray(matrices, np.float64).reshape(-1),
                                 np.ascontiguousarray(frequencies, np.float64)))
+
+    def propose_propagations(self, nodes, children, num_leaves, matrices, frequencies):
+        """CudaPropagationsCalculator::propose: propagation lists and one
+        matrix per listed node but the root, [K][len(nodes) - 1][4][4]."""
+        nodes = np.ascontiguousarray(nodes, np.uint32)
+        check(lib().b3h_propose_propagations(
+            self._h, nodes.size, nodes, int(num_leaves),
+            np.ascontiguousarray(children, np.uint32).reshape(-1),
+            np.ascontiguousarray(matrices, np.float64).reshape(-1),
+            np.ascontiguousarray(frequencies, np.float64)))
+
+    def propose_propagations_times(self, nodes, children, num_leaves, edge_times,
+                                   category_rates):
+        """edge_times: time length of each listed node's edge but the root's."""
+        nodes = np.ascontiguousarray(nodes, np.uint32)
+        check(lib().b3h_propose_propagations_times(
+            self._h, nodes.size, nodes, int(num_leaves),
+            np.ascontiguousarray(children, np.uint32).reshape(-1),
+            np.ascontiguousarray(edge_times, np.float64),
+            np.ascontiguousarray(category_rates, np.float64)))
 
     def propose_times(self, internals, children, edge_times, category_rates):
         internals = np.ascontiguousarray(internals, np.uint32)

@@ -106,9 +106,21 @@ class GammaLikelihood:
         if self._alpha_changed:
             tree.mark_all_edges_updated()
             self._rates = self._category_rates(self.alpha)
+        self._launched = True
+        if self.calculator.propagations:
+            # b3's propagation calculator takes its own lists and one matrix
+            # per listed node but the root (cuda_propagations/mod.rs:77-86)
+            if self.clock is not None and not self.clock.strict:
+                raise NotImplementedError("propagation storage takes a strict clock")
+            nodes, pchildren, nl = tree.propagation_lists()
+            times, _ = _child_times(tree, nodes[:-1])
+            rate = self.clock_rate if self.clock is None else self.clock.get_rate()
+            self.calculator.propose_propagations_times(nodes, pchildren, nl, times,
+                                                       rate * self._rates)
+            self._last = self.calculator.total()
+            return self._last
         internals, children = tree.partials_lists()
         times, ch = _child_times(tree, children)
-        self._launched = True
         if self.clock is None or self.clock.strict:
             # time x (clock rate x category rate), gamma.rs:128-134
             rate = self.clock_rate if self.clock is None else self.clock.get_rate()

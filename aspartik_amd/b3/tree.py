@@ -3,7 +3,8 @@
 Mirrors the parts of ``DatedTree`` (crates/b3/src/parameters/tree.rs) that
 feed the likelihood: node numbering (leaves 0..L-1, internals L..2L-2, edge
 index = child id, tree.rs:728-743), ``mark_edge_updated`` (357-375),
-``mark_all_edges_updated`` (382-386), ``partials_lists`` (398-461),
+``mark_all_edges_updated`` (382-386), ``partials_lists`` and
+``propagation_lists`` (398-461),
 ``set_height`` (516-528), ``replace_child`` (471-495) and accept/reject
 (975-995).
 """
@@ -30,6 +31,7 @@ class DatedTree:
         self.root = int(roots[0])
         self._upd_edges = np.zeros(self.N, bool)
         self._upd_partials = np.zeros(self.N, bool)
+        self._upd_props = np.zeros(self.N, bool)
         self._save()
 
     # ---- structure -------------------------------------------------------
@@ -58,18 +60,22 @@ class DatedTree:
 
     # ---- dirty tracking ---------------------------------------------------
     def mark_edge_updated(self, edge):
-        """tree.rs:357-375: the edge, then ancestors up to the first marked."""
+        """tree.rs:357-375: the edge and its child node's projection, then
+        ancestors up to the first marked."""
         self._upd_edges[edge] = True
+        self._upd_props[edge] = True
         curr = int(edge)
         while self.parents[curr] >= 0:
             curr = int(self.parents[curr])
             if self._upd_partials[curr]:
                 break
+            self._upd_props[curr] = True
             self._upd_partials[curr] = True
 
     def mark_all_edges_updated(self):
         self._upd_edges[:] = True
         self._upd_partials[:] = True
+        self._upd_props[:] = True
 
     def is_changed(self):
         return bool(self._upd_edges.any())
@@ -77,23 +83,39 @@ class DatedTree:
     def edges_to_update(self):
         return [e for e in self.edges() if self._upd_edges[e]]
 
-    def partials_lists(self):
-        """tree.rs:398-434,456-461: BFS from the root through marked internal
-        children, reversed (children first, root last); sorted child pairs."""
+    def _updated_internals(self, bitmap):
+        """tree.rs:398-421: BFS from the root through marked internal
+        children, reversed (children first, root last)."""
         out = [self.root]
         q = deque([self.root])
         while q:
             node = q.popleft()
             left, right = self.children_of(node)
             for ch in (left, right):
-                if self._upd_partials[ch] and ch >= self.L:
+                if bitmap[ch] and ch >= self.L:
                     out.append(ch)
                     q.append(ch)
         out.reverse()
-        internals = np.asarray(out, np.uint32)
+        return np.asarray(out, np.uint32)
+
+    def _children_list(self, internals):
+        """tree.rs:423-434: each pair sorted ascending."""
         ch = self.children[internals.astype(np.int64) - self.L]
-        children = np.sort(ch, axis=1).astype(np.uint32)
-        return internals, children
+        return np.sort(ch, axis=1).astype(np.uint32)
+
+    def partials_lists(self):
+        """tree.rs:456-461"""
+        internals = self._updated_internals(self._upd_partials)
+        return internals, self._children_list(internals)
+
+    def propagation_lists(self):
+        """tree.rs:436-454: (nodes, children, number of updated leaves); the
+        updated leaves in order, then the internal nodes whose projections
+        change, children first, root last."""
+        leaves = np.nonzero(self._upd_props[:self.L])[0].astype(np.uint32)
+        internals = self._updated_internals(self._upd_props)
+        nodes = np.concatenate([leaves, internals]).astype(np.uint32)
+        return nodes, self._children_list(internals), int(leaves.size)
 
     # ---- edits -----------------------------------------------------------
     def set_height(self, node, height):
@@ -129,6 +151,7 @@ class DatedTree:
     def _clear(self):
         self._upd_edges[:] = False
         self._upd_partials[:] = False
+        self._upd_props[:] = False
 
     def accept(self):
         self._save()
@@ -148,6 +171,7 @@ class DatedTree:
         t.root = self.root
         t._upd_edges = self._upd_edges.copy()
         t._upd_partials = self._upd_partials.copy()
+        t._upd_props = self._upd_props.copy()
         t._save()
         return t
 

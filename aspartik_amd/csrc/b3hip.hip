@@ -176,6 +176,10 @@ b3h_status set_device(const b3h_calc *c) {
   return B3H_OK;
 }
 
+bool is_prop(const b3h_calc *c) {
+  return c->precision == B3H_F64_PROPAGATION || c->precision == B3H_F32_PROPAGATION;
+}
+
 // Slot the result reads use: the pending propose's, else the accepted one.
 b3h_status result_slot(const b3h_calc *c, uint32_t *slot) {
   if (c->pending) *slot = c->rslot ^ 1u;
@@ -443,6 +447,142 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
   return B3H_OK;
 }
 
+// CudaPropagationsCalculator::propose (cuda_propagations/mod.rs:77-141):
+// selectors of every listed node are edited, one transition per node but the
+// root, leaves' projections, internal nodes children first, the root.
+b3h_status run_propose_prop(b3h_calc *c, uint32_t nn, const uint32_t *nodes, uint32_t nl,
+                            const uint32_t *children, const double *mats, const double *times,
+                            const double *rates, const double *freqs) {
+  b3h_status st = set_device(c);
+  if (st) return st;
+  const uint32_t L = c->L, N = 2 * L - 1, K = c->K;
+  if (nn == 0) return fail(B3H_ERR_ARG, "empty update list");
+  if (nn > N) return fail(B3H_ERR_ARG, "update list longer than the node count");
+  if (!nodes || !children) return fail(B3H_ERR_ARG, "null update list");
+  if (nl >= nn) return fail(B3H_ERR_ARG, "the list must end with the root, an internal node");
+  for (uint32_t i = 0; i < nn; i++) {
+    const bool leaf = i < nl;
+    if (leaf ? nodes[i] >= L : (nodes[i] < L || nodes[i] >= N))
+      return fail(B3H_ERR_ARG, "node id out of range at " + std::to_string(i));
+  }
+  for (uint32_t i = 0; i < 2 * (nn - nl); i++)
+    if (children[i] >= N || children[i] == nodes[nl + i / 2])
+      return fail(B3H_ERR_ARG, "child node id out of range at " + std::to_string(i / 2));
+  // EditBuf::set_edited of every listed node (mod.rs:91-93)
+  c->editbuf_next = c->editbuf;
+  for (uint32_t i = 0; i < nn; i++) {
+    uint8_t &m = c->editbuf_next[nodes[i]];
+    m = (uint8_t)(((m & 1u) ^ (uint8_t)~(m >> 1)) & 3u);
+  }
+  c->editbuf.swap(c->editbuf_next);
+  auto slot = [&](uint32_t node) { return node * 2u + (c->editbuf[node] & 1u); };
+
+  const uint32_t nm = nn - 1;
+  const int mode = mats ? 0 : 1;
+  const size_t ebytes = (size_t)nn * sizeof(Entry);
+  const size_t mbytes = mats ? (size_t)K * nm * 16 * 8 : 0;
+  const size_t tbytes = times ? (size_t)nm * 8 : 0;
+  const size_t rbytes = rates ? (size_t)K * 8 : 0;
+  const size_t off_m = (ebytes + 255) & ~(size_t)255;
+  const size_t off_t = off_m + ((mbytes + 255) & ~(size_t)255);
+  const size_t off_r = off_t + ((tbytes + 255) & ~(size_t)255);
+  const size_t off_pi = off_r + ((rbytes + 255) & ~(size_t)255);
+  const size_t total = off_pi + 64 * 8;
+  if (total > c->stage_bytes) return fail(B3H_ERR_ARG, "staging overflow");
+  int sidx = 0;
+  unsigned char *h = next_stage(c, &sidx);
+  Entry *ent = reinterpret_cast<Entry *>(h);
+  for (uint32_t i = 0; i < nn; i++) {
+    Entry e{};
+    e.out = slot(nodes[i]);
+    if (i < nl) {
+      e.left = nodes[i] | SRC_TIP;
+    } else {
+      e.left = slot(children[2 * (i - nl)]);
+      e.right = slot(children[2 * (i - nl) + 1]);
+    }
+    e.meta = i << 8;
+    ent[i] = e;
+  }
+  if (mats) std::memcpy(h + off_m, mats, mbytes);
+  if (times) std::memcpy(h + off_t, times, tbytes);
+  if (rates) std::memcpy(h + off_r, rates, rbytes);
+  double *hpi = reinterpret_cast<double *>(h + off_pi);
+  for (uint32_t j = 0; j < 64; j++) hpi[j] = j < 4 ? freqs[j] : 0.0;
+  unsigned char *d = c->d_stage[sidx];
+  HIPCHK(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipEventRecord(c->stage_ev[sidx], c->stream));
+  if (mode == 0)
+    for (uint32_t j = 0; j < 4; j++) c->freqs[j] = freqs[j];
+
+  const bool f64 = c->precision == B3H_F64_PROPAGATION;
+  const double *d_mats = reinterpret_cast<const double *>(d + off_m);
+  const double *d_times = reinterpret_cast<const double *>(d + off_t);
+  const double *d_rates = reinterpret_cast<const double *>(d + off_r);
+  if (f64)
+    HIPCHK(launch_pbuild_prop<double>(nm, K, mode, d_mats, d_times, d_rates, c->d_model,
+                                      static_cast<double *>(c->d_pmats), c->stream));
+  else
+    HIPCHK(launch_pbuild_prop<float>(nm, K, mode, d_mats, d_times, d_rates, c->d_model,
+                                     static_cast<float *>(c->d_pmats), c->stream));
+
+  const uint32_t ws = c->rslot ^ 1u;
+  double *r_site = c->d_site + (size_t)ws * K * c->P;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->profiling) {
+    if (c->ev_used == c->ev_start.size()) {
+      hipEvent_t ea, eb;
+      HIPCHK(hipEventCreate(&ea));
+      HIPCHK(hipEventCreate(&eb));
+      c->ev_start.push_back(ea);
+      c->ev_stop.push_back(eb);
+    }
+    e0 = c->ev_start[c->ev_used];
+    e1 = c->ev_stop[c->ev_used];
+    c->ev_used++;
+  }
+  PropArgs a{};
+  a.entries = reinterpret_cast<const Entry *>(d);
+  a.mats = c->d_pmats;
+  a.codes = c->d_codes;
+  a.proj = c->d_partials;
+  a.scales = c->d_scales;
+  a.exps = c->d_exps;
+  a.scale_sums = c->d_scale_sums;
+  a.site = r_site;
+  a.pi0 = freqs[0];
+  a.pi1 = freqs[1];
+  a.pi2 = freqs[2];
+  a.pi3 = freqs[3];
+  a.pik = nullptr;
+  a.scale_ln = c->scale_ln;
+  a.threshold = std::exp(-(double)c->scale_ln);  // mod.rs:267-268
+  a.mult = std::exp((double)c->scale_ln);
+  a.n = nn;
+  a.nleaves = nl;
+  a.P = c->P;
+  a.K = K;
+  a.cstride = c->cstride;
+  if (e0) HIPCHK(hipEventRecord(e0, c->stream));
+  HIPCHK(launch_prune_prop(a, f64, c->stream));
+  if (e1) HIPCHK(hipEventRecord(e1, c->stream));
+  MixArgs m{};
+  m.site = r_site;
+  m.classes = c->classes_on ? c->d_classes : nullptr;
+  m.mixed = c->d_mixed + (size_t)ws * c->P;
+  m.weights = c->d_weights;
+  m.blocksum = c->d_mix_blocksum;
+  m.counter = c->d_counter;
+  m.total = c->d_total + ws;
+  m.ln_inv_k = std::log(1.0 / (double)K);
+  m.P = c->P;
+  m.K = K;
+  HIPCHK(launch_mix(m, c->stream));
+  c->has_result = true;
+  c->pending = true;
+  return B3H_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -483,13 +623,14 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
     return fail(B3H_ERR_ARG, "num_categories must be 1..16");
   if (cfg->scale_ln == 0) return fail(B3H_ERR_ARG, "scale_ln must be nonzero");
   if (cfg->precision != B3H_F32_SCALED && cfg->precision != B3H_F64_PLAIN &&
-      cfg->precision != B3H_F16_MFMA)
+      cfg->precision != B3H_F16_MFMA && cfg->precision != B3H_F64_PROPAGATION &&
+      cfg->precision != B3H_F32_PROPAGATION)
     return fail(B3H_ERR_UNSUPPORTED, "unknown precision");
   if (cfg->precision == B3H_F16_MFMA) {
     if (cfg->states < 2 || cfg->states > 64)
       return fail(B3H_ERR_UNSUPPORTED, "the MFMA path supports 2..64 states");
   } else if (cfg->states != 4) {
-    return fail(B3H_ERR_UNSUPPORTED, "the FP32 and FP64 paths support 4 states");
+    return fail(B3H_ERR_UNSUPPORTED, "the FP32, FP64 and propagation paths support 4 states");
   }
   if ((uint64_t)num_leaves * 2 >= (1u << 24))
     return fail(B3H_ERR_ARG, "too many leaves");
@@ -508,7 +649,10 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   c->K = num_categories;
   c->S = cfg->states;
   c->scale_ln = cfg->scale_ln;
-  c->editbuf.assign(c->I, 0);
+  // partial storage: one EditBuf entry per internal node; propagation
+  // storage: one selector per node (cuda_propagations/mod.rs:327)
+  const bool prop = c->precision == B3H_F64_PROPAGATION || c->precision == B3H_F32_PROPAGATION;
+  c->editbuf.assign(prop ? 2 * c->L - 1 : c->I, 0);
   c->has_result = false;
 
   auto bail = [&](b3h_status s) {
@@ -527,12 +671,20 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   }
 
   const uint32_t K = c->K, P = c->P, S = c->S;
-  const size_t slots = (size_t)2 * c->I;
+  const size_t slots = prop ? (size_t)2 * (2 * c->L - 1) : (size_t)2 * c->I;
   const bool f32 = c->precision == B3H_F32_SCALED;
   const bool codon = c->precision == B3H_F16_MFMA;
+  const bool f32_prop = c->precision == B3H_F32_PROPAGATION;
   b3h_status st;
 
-  if (codon) {
+  if (prop) {
+    // propagation storage: one lane per (pattern, category), no tiles
+    c->T = 1;
+    c->PPL = 1;
+    c->cstride = (P + 63) / 64 * 64;
+    c->blocks = 1;
+    c->sched.capacity = 0;
+  } else if (codon) {
     // MFMA path: 32-pattern tiles, four per 256-thread workgroup, patterns
     // padded to a multiple of 128; FP16 partials of 64 states; no LDS stack
     // (children not in registers are read back from HBM)
@@ -580,9 +732,9 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   if ((st = dev_alloc(c, (void **)&c->d_codes, codes.size()))) return bail(st);
   // partial store, one "pattern unit" per (slot, category, pattern)
   const size_t punits = codon ? slots * K * c->cstride : slots * K * P;
-  const size_t elem = codon ? 128 : f32 ? 16 : 32;
+  const size_t elem = codon ? 128 : (f32 || f32_prop) ? 16 : 32;
   if ((st = dev_alloc(c, &c->d_partials, punits * elem))) return bail(st);
-  if (f32 || codon) {
+  if (f32 || codon || f32_prop) {
     if ((st = dev_alloc(c, (void **)&c->d_exps, punits * 2))) return bail(st);
   } else {
     if ((st = dev_alloc(c, (void **)&c->d_scales, slots * K * P))) return bail(st);
@@ -598,7 +750,7 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   if ((st = dev_alloc(c, (void **)&c->d_mix_blocksum, (size_t)mix_blocks(P) * 8))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_counter, 64))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_total, 64))) return bail(st);
-  const size_t mat_bytes = codon ? 8192 * 2 : (size_t)S * S * (f32 ? 4 : 8);
+  const size_t mat_bytes = codon ? 8192 * 2 : (size_t)S * S * ((f32 || f32_prop) ? 4 : 8);
   if ((st = dev_alloc(c, &c->d_pmats, (size_t)c->I * 2 * K * mat_bytes))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_model, (size_t)K * sizeof(DevModel)))) return bail(st);
   if (codon && (st = dev_alloc(c, (void **)&c->d_ucodon, (size_t)64 * 4096 * 8))) return bail(st);
@@ -606,7 +758,7 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   // staging: entries + host matrices (the larger of the f64 input and the
   // FP16 A layout) + times + rates + stationary frequencies
   const size_t host_mat = std::max<size_t>((size_t)S * S * 8, codon ? 8192 * 2 : 0);
-  c->stage_bytes = ((size_t)c->I * sizeof(Entry) + 255) / 256 * 256 +
+  c->stage_bytes = ((size_t)2 * c->L * sizeof(Entry) + 255) / 256 * 256 +
                    ((size_t)K * 2 * c->I * host_mat + 255) / 256 * 256 +
                    ((size_t)K * 2 * c->I * 8 + 255) / 256 * 256 + ((size_t)K * 8 + 255) / 256 * 256 +
                    64 * 8 + 256;
@@ -629,8 +781,8 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
             hipMemset(c->d_model, 0, (size_t)K * sizeof(DevModel)) == hipSuccess &&
             hipMemset(c->d_classes, 0, P) == hipSuccess &&
             hipMemset(c->d_partials, 0, punits * elem) == hipSuccess;
-  if (ok && (f32 || codon)) ok = hipMemset(c->d_exps, 0, punits * 2) == hipSuccess;
-  if (ok && !f32 && !codon)
+  if (ok && (f32 || codon || f32_prop)) ok = hipMemset(c->d_exps, 0, punits * 2) == hipSuccess;
+  if (ok && !f32 && !codon && !f32_prop)
     ok = hipMemset(c->d_scales, 0, slots * K * P) == hipSuccess &&
          hipMemset(c->d_scale_sums, 0, (size_t)K * P * 4) == hipSuccess &&
          hipMemset(c->d_scale_sums_bak, 0, (size_t)K * P * 4) == hipSuccess;
@@ -671,7 +823,36 @@ b3h_status b3h_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
   b3h_status st = check_handle(c);
   if (st) return st;
   if (!matrices || !frequencies) return fail(B3H_ERR_ARG, "null matrices or frequencies");
+  if (is_prop(c)) return fail(B3H_ERR_UNSUPPORTED, "propagation storage takes propagation lists");
   return run_propose(c, n, internals, children, matrices, nullptr, nullptr, frequencies);
+}
+
+b3h_status b3h_propose_propagations(b3h_calc *c, uint32_t num_nodes, const uint32_t *nodes,
+                                    uint32_t num_leaves, const uint32_t *children,
+                                    const double *matrices, const double *frequencies) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if (!is_prop(c)) return fail(B3H_ERR_UNSUPPORTED, "the handle does not use propagation storage");
+  if (!matrices || !frequencies) return fail(B3H_ERR_ARG, "null matrices or frequencies");
+  return run_propose_prop(c, num_nodes, nodes, num_leaves, children, matrices, nullptr, nullptr,
+                          frequencies);
+}
+
+b3h_status b3h_propose_propagations_times(b3h_calc *c, uint32_t num_nodes,
+                                          const uint32_t *nodes, uint32_t num_leaves,
+                                          const uint32_t *children, const double *edge_times,
+                                          const double *category_rates) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if (!is_prop(c)) return fail(B3H_ERR_UNSUPPORTED, "the handle does not use propagation storage");
+  if (!c->model_set) return fail(B3H_ERR_STATE, "b3h_set_model must be called first");
+  if (!edge_times || !category_rates) return fail(B3H_ERR_ARG, "null edge times or rates");
+  if (c->per_category)
+    return fail(B3H_ERR_UNSUPPORTED, "propagation storage uses one model for all categories");
+  double fr[4];
+  for (uint32_t j = 0; j < 4; j++) fr[j] = c->freqs[j];
+  return run_propose_prop(c, num_nodes, nodes, num_leaves, children, nullptr, edge_times,
+                          category_rates, fr);
 }
 
 b3h_status b3h_set_model(b3h_calc *c, const double *q, const double *freqs,
@@ -708,6 +889,8 @@ b3h_status b3h_set_category_model(b3h_calc *c, uint32_t category, const double *
   if (category >= c->K) return fail(B3H_ERR_ARG, "category out of range");
   if (c->precision == B3H_F16_MFMA)
     return fail(B3H_ERR_UNSUPPORTED, "per-category models need 4 states");
+  if (is_prop(c))
+    return fail(B3H_ERR_UNSUPPORTED, "propagation storage uses one model for all categories");
   for (uint32_t i = 0; i < 4; i++)
     if (!(freqs[i] > 0.0)) return fail(B3H_ERR_ARG, "frequencies must be positive");
   if ((st = set_device(c))) return st;
@@ -752,8 +935,9 @@ b3h_status b3h_propose_lengths(b3h_calc *c, uint32_t n, const uint32_t *internal
   if (st) return st;
   if (!c->model_set) return fail(B3H_ERR_STATE, "b3h_set_model must be called first");
   if (!lengths) return fail(B3H_ERR_ARG, "null branch lengths");
-  if (c->precision == B3H_F16_MFMA)
-    return fail(B3H_ERR_UNSUPPORTED, "per-category branch lengths need 4 states");
+  if (c->precision == B3H_F16_MFMA || is_prop(c))
+    return fail(B3H_ERR_UNSUPPORTED,
+                "per-category branch lengths need the FP32 or FP64 partial paths");
   double fr[64];
   for (uint32_t j = 0; j < c->S; j++) fr[j] = c->freqs[j];
   return run_propose(c, n, internals, children, nullptr, lengths, nullptr, fr);
@@ -805,6 +989,7 @@ b3h_status b3h_propose_times(b3h_calc *c, uint32_t n, const uint32_t *internals,
   if (st) return st;
   if (!c->model_set) return fail(B3H_ERR_STATE, "b3h_set_model must be called first");
   if (!edge_times || !category_rates) return fail(B3H_ERR_ARG, "null edge times or rates");
+  if (is_prop(c)) return fail(B3H_ERR_UNSUPPORTED, "propagation storage takes propagation lists");
   double fr[64];
   for (uint32_t j = 0; j < c->S; j++) fr[j] = c->freqs[j];
   return run_propose(c, n, internals, children, nullptr, edge_times, category_rates, fr);
@@ -819,7 +1004,7 @@ b3h_status b3h_accept(b3h_calc *c) {
     c->accepted_valid = true;
     c->pending = false;
   }
-  if (c->precision == B3H_F64_PLAIN) {  // cpu4.rs:78-83
+  if (c->precision == B3H_F64_PLAIN || c->precision == B3H_F64_PROPAGATION) {  // cpu4.rs:78-83
     if ((st = set_device(c))) return st;
     HIPCHK(hipMemcpyAsync(c->d_scale_sums_bak, c->d_scale_sums, (size_t)c->K * c->P * 4,
                           hipMemcpyDeviceToDevice, c->stream));
@@ -832,7 +1017,7 @@ b3h_status b3h_reject(b3h_calc *c) {
   if (st) return st;
   for (auto &m : c->editbuf) m = (uint8_t)((m ^ (m >> 1)) & 1u);  // editbuf.rs:55-66
   c->pending = false;
-  if (c->precision == B3H_F64_PLAIN) {  // cpu4.rs:85-89
+  if (c->precision == B3H_F64_PLAIN || c->precision == B3H_F64_PROPAGATION) {  // cpu4.rs:85-89
     if ((st = set_device(c))) return st;
     HIPCHK(hipMemcpyAsync(c->d_scale_sums, c->d_scale_sums_bak, (size_t)c->K * c->P * 4,
                           hipMemcpyDeviceToDevice, c->stream));

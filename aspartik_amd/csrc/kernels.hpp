@@ -110,6 +110,34 @@ struct F64Args {
 };
 hipError_t launch_prune_f64(const F64Args &a, hipStream_t s);
 
+// ---- prune_prop.hip ------------------------------------------------------
+// Propagation storage: per-node projections [2N slots][K][P] of double4
+// (b3's flag scaling) or float4 (binary exponents).  Entries: updated leaves
+// (out slot, left = leaf id), internal nodes (out, left and right child
+// slots), the root last (out = its slot, children slots).
+struct PropArgs {
+  const Entry *entries;
+  const void *mats;         // [n - 1][K][16] of double or float, entry order
+  const uint8_t *codes;     // [L][cstride] state codes 0..4
+  void *proj;               // [2N][K][P] double4 or float4
+  uint8_t *scales;          // double: [2N][K][P] flags
+  int16_t *exps;            // float: [2N][K][P] cumulative exponents
+  uint32_t *scale_sums;     // double: [K][P]
+  double *site;             // [K][P]
+  double pi0, pi1, pi2, pi3;
+  const double *pik;        // [K][4] or null
+  double threshold, mult;
+  uint32_t scale_ln;
+  uint32_t n, nleaves, P, K, cstride;
+};
+hipError_t launch_prune_prop(const PropArgs &a, bool f64, hipStream_t s);
+// One transition per listed node but the root, [m][K][16]: mode 0 casts host
+// matrices [K][nm][16]; mode 1 builds P(times[m] * rates[c]).
+template <typename T>
+hipError_t launch_pbuild_prop(uint32_t nm, uint32_t K, int mode, const double *mats,
+                              const double *times, const double *rates, const DevModel *model,
+                              T *out, hipStream_t s);
+
 // Category mix + weighted sum with a deterministic last-block reduction.
 // With classes, pattern i takes category classes[i]'s value instead of the
 // mean over categories.

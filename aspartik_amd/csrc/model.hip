@@ -169,6 +169,47 @@ hipError_t launch_pbuild_batch(const PbuildDesc *desc, uint32_t count, uint32_t 
   return hipGetLastError();
 }
 
+// One transition per propagation-list node (its own edge), [m][K][16].
+template <typename T>
+__global__ void pbuild_prop_kernel(uint32_t nm, uint32_t K, int mode, const double *mats,
+                                   const double *times, const double *rates,
+                                   const DevModel *model, T *out) {
+  const uint64_t total = (uint64_t)nm * K * 16;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t el = (uint32_t)(g % 16);
+    const uint64_t mc = g / 16;
+    const uint32_t c = (uint32_t)(mc % K);
+    const uint32_t m = (uint32_t)(mc / K);
+    double v;
+    if (mode == 0)
+      v = mats[((uint64_t)c * nm + m) * 16 + el];
+    else
+      v = p_element(model, 4, times[m] * rates[c], (int)(el / 4), (int)(el % 4));
+    out[g] = (T)v;
+  }
+}
+
+template <typename T>
+hipError_t launch_pbuild_prop(uint32_t nm, uint32_t K, int mode, const double *mats,
+                              const double *times, const double *rates, const DevModel *model,
+                              T *out, hipStream_t s) {
+  const uint64_t total = (uint64_t)nm * K * 16;
+  uint64_t blocks = (total + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(pbuild_prop_kernel<T>, dim3((uint32_t)blocks), dim3(256), 0, s, nm, K,
+                     mode, mats, times, rates, model, out);
+  return hipGetLastError();
+}
+
+template hipError_t launch_pbuild_prop<float>(uint32_t, uint32_t, int, const double *,
+                                              const double *, const double *, const DevModel *,
+                                              float *, hipStream_t);
+template hipError_t launch_pbuild_prop<double>(uint32_t, uint32_t, int, const double *,
+                                               const double *, const double *, const DevModel *,
+                                               double *, hipStream_t);
+
 hipError_t launch_eigen(DevModel *m, hipStream_t s) {
   hipLaunchKernelGGL(eigen_kernel, dim3(1), dim3(64), 0, s, m);
   return hipGetLastError();

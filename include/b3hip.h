@@ -51,7 +51,15 @@ typedef enum b3h_precision {
   B3H_F64_PLAIN = 1,
   /* Codon / S-state models: partials FP16 in HBM, P x partials on MFMA
    * (fp16 in, fp32 accumulate), power-of-two scaling. */
-  B3H_F16_MFMA = 2
+  B3H_F16_MFMA = 2,
+  /* Propagation storage, b3's default GPU algorithm (cuda_propagations/
+   * kernels.cu:73-187) at FP64: every node keeps the projection P x of its
+   * likelihood in two selector slots, e^scale_ln flag scaling of any node.
+   * Takes propagation lists (b3h_propose_propagations*). */
+  B3H_F64_PROPAGATION = 3,
+  /* Propagation storage with FP32 projections and power-of-two exponents,
+   * FP64 root and sums: the storage alternative to B3H_F32_SCALED. */
+  B3H_F32_PROPAGATION = 4
 } b3h_precision;
 
 typedef struct b3h_config {
@@ -181,6 +189,34 @@ b3h_status b3h_propose_lengths(b3h_calc *calc, uint32_t n,
  * hetero.rs:117-120,150-160 path when only the classes changed.
  */
 b3h_status b3h_set_classes(b3h_calc *calc, const uint32_t *classes);
+
+/*
+ * CudaPropagationsCalculator::propose (calculator/cuda_propagations/mod.rs:
+ * 77-141), for handles created with a propagation precision.
+ *   nodes[num_nodes], children[num_nodes - num_leaves][2], num_leaves:
+ *       DatedTree::propagation_lists() (tree.rs:436-454): the updated
+ *       leaves, then the internal nodes whose projections change, children
+ *       first, the root last; child pairs sorted.
+ *   matrices: [K][num_nodes - 1][4][4] f64, one per listed node but the
+ *       root, Transitions::write_matrices(&nodes[..len - 1]) order.
+ *   frequencies: [4].
+ * The _times form builds the matrices on the GPU from the device model:
+ *   edge_times[num_nodes - 1] (the time length of each listed node's edge)
+ *   times category_rates[K].
+ * Other propose calls return B3H_ERR_UNSUPPORTED on such handles, and these
+ * return it on the others.
+ */
+b3h_status b3h_propose_propagations(b3h_calc *calc, uint32_t num_nodes,
+                                    const uint32_t *nodes, uint32_t num_leaves,
+                                    const uint32_t *children,
+                                    const double *matrices,
+                                    const double *frequencies);
+b3h_status b3h_propose_propagations_times(b3h_calc *calc, uint32_t num_nodes,
+                                          const uint32_t *nodes,
+                                          uint32_t num_leaves,
+                                          const uint32_t *children,
+                                          const double *edge_times,
+                                          const double *category_rates);
 
 /* Calculator::accept / reject (calculator/mod.rs:62-69; cpu4.rs:78-89).
  * Results are kept in two slots like the likelihood layer's buffer

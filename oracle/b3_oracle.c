@@ -40,6 +40,15 @@
  *   Gamma mixing                gamma.rs:154-166 - pinned: brute force.
  *   deduplicate                 likelihood/mod.rs:74-132
  *       pinned: first-occurrence order, counts, round trip.
+ *   Components from scratch     likelihood/hetero.rs:122-160, gamma.rs:126-166
+ *       with per-edge clock rates (clock.rs:115-119, transitions.rs:33-35);
+ *       pinned: brute force with scipy expm, strict-clock and one-category
+ *       reductions.
+ *   Propagation calculator      calculator/cuda_propagations/kernels.cu:73-187,
+ *       mod.rs:77-164 (b3's default GPU backend) restated on the CPU: stored
+ *       per-node projections P x, per-node-slot flags with scaling of any node
+ *       (cherries included), the root's flag handling; pinned: equals Cpu4 on
+ *       full and partial updates (1e-12), brute force, scaling invariance.
  */
 #define _GNU_SOURCE
 #include "b3_oracle.h"
@@ -1155,6 +1164,138 @@ const double *or_gamma_lik_site(const or_gamma_lik *g, int c) {
 }
 
 const double *or_gamma_lik_rates(const or_gamma_lik *g) { return g->rates; }
+
+/* ======================================================================== */
+/* Propagation calculator (cuda_propagations): every node n stores, in one of
+ * two slots chosen by its selector (an EditBuf over all nodes), the
+ * projection P(edge n) x_n of its likelihood x_n (kernels.cu:22-59).  An
+ * update list from DatedTree::propagation_lists (tree.rs:436-454) names the
+ * updated leaves, then the internal nodes children first, root last; one
+ * transition per listed node but the root (mod.rs:83-86).  Internal node:
+ * x = proj(left) * proj(right); if all four values < e^-scale_ln the node is
+ * scaled by e^scale_ln and flagged (kernels.cu:110-136); then
+ * proj = P x (kernels.cu:138-150).  Root: ln(pi . x) - scale_sum, with the
+ * root's own flag cleared (kernels.cu:155-187). */
+struct or_prop4 {
+  size_t P;
+  uint32_t L, N, scale_ln;
+  double threshold, mult;
+  uint8_t *leaves;     /* [L][P] */
+  double *proj;        /* [2N][P][4] */
+  uint8_t *scales;     /* [2N][P] */
+  uint32_t *ss, *ss_bak;
+  uint8_t *sel;        /* [N] */
+  double *lik;         /* [P] */
+};
+
+or_prop4 *or_prop4_new(size_t P, const uint8_t *samples, uint32_t L, uint32_t scale_ln) {
+  if (L < 2 || P == 0) return NULL;
+  or_prop4 *c = calloc(1, sizeof *c);
+  c->P = P;
+  c->L = L;
+  c->N = 2 * L - 1;
+  c->scale_ln = scale_ln;
+  c->threshold = exp(-(double)scale_ln); /* mod.rs:267-268 */
+  c->mult = exp((double)scale_ln);
+  c->leaves = malloc((size_t)L * P);
+  memcpy(c->leaves, samples, (size_t)L * P);
+  c->proj = calloc((size_t)2 * c->N * P * 4, sizeof(double));
+  c->scales = calloc((size_t)2 * c->N * P, 1);
+  c->ss = calloc(P, 4);
+  c->ss_bak = calloc(P, 4);
+  c->sel = calloc(c->N, 1);
+  c->lik = calloc(P, 8);
+  return c;
+}
+
+void or_prop4_free(or_prop4 *c) {
+  if (!c) return;
+  free(c->leaves); free(c->proj); free(c->scales);
+  free(c->ss); free(c->ss_bak); free(c->sel); free(c->lik);
+  free(c);
+}
+
+static inline size_t prop_slot(const or_prop4 *c, uint32_t node) {
+  return (size_t)node * 2 + (c->sel[node] & 1u);
+}
+
+/* kernels.cu:5-7: a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w, left to right */
+static inline double dot4(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+}
+
+void or_prop4_propose(or_prop4 *c, uint32_t num_nodes, const uint32_t *nodes,
+                      uint32_t num_leaves, const uint32_t *children,
+                      const double *mats, const double *freqs) {
+  const size_t P = c->P;
+  for (uint32_t i = 0; i < num_nodes; i++) or_editbuf_set_edited(c->sel, nodes[i]);
+  for (uint32_t i = 0; i < num_leaves; i++) { /* CALCULATE_LEAF_PROJECTION */
+    const uint32_t leaf = nodes[i];
+    const double *M = mats + (size_t)i * 16;
+    double *dst = c->proj + prop_slot(c, leaf) * P * 4;
+    for (size_t p = 0; p < P; p++) {
+      const uint8_t v = c->leaves[(size_t)leaf * P + p];
+      for (int s = 0; s < 4; s++) {
+        double x = 1.0;
+        if (v == 1) x = M[s * 4 + 0];
+        else if (v == 2) x = M[s * 4 + 1];
+        else if (v == 4) x = M[s * 4 + 2];
+        else if (v == 8) x = M[s * 4 + 3];
+        dst[p * 4 + s] = x;
+      }
+    }
+  }
+  for (uint32_t i = num_leaves; i + 1 < num_nodes; i++) {
+    const uint32_t node = nodes[i];
+    const uint32_t l = children[2 * (i - num_leaves)], r = children[2 * (i - num_leaves) + 1];
+    const double *M = mats + (size_t)i * 16;
+    const size_t snew = prop_slot(c, node), sold = snew ^ 1u;
+    const double *pl = c->proj + prop_slot(c, l) * P * 4;
+    const double *pr = c->proj + prop_slot(c, r) * P * 4;
+    double *dst = c->proj + snew * P * 4;
+    for (size_t p = 0; p < P; p++) {
+      double x[4];
+      for (int s = 0; s < 4; s++) x[s] = pl[p * 4 + s] * pr[p * 4 + s];
+      const int scale = x[0] < c->threshold && x[1] < c->threshold &&
+                        x[2] < c->threshold && x[3] < c->threshold;
+      if (scale)
+        for (int s = 0; s < 4; s++) x[s] *= c->mult;
+      const uint8_t old = c->scales[sold * P + p];
+      c->scales[snew * P + p] = (uint8_t)scale;
+      if (old == 0 && scale) c->ss[p] += c->scale_ln;
+      else if (old == 1 && !scale) c->ss[p] -= c->scale_ln;
+      for (int s = 0; s < 4; s++) dst[p * 4 + s] = dot4(M + s * 4, x);
+    }
+  }
+  { /* update_likelihoods */
+    const uint32_t root = nodes[num_nodes - 1];
+    const uint32_t l = children[2 * (num_nodes - 1 - num_leaves)];
+    const uint32_t r = children[2 * (num_nodes - 1 - num_leaves) + 1];
+    const size_t snew = prop_slot(c, root), sold = snew ^ 1u;
+    const double *pl = c->proj + prop_slot(c, l) * P * 4;
+    const double *pr = c->proj + prop_slot(c, r) * P * 4;
+    for (size_t p = 0; p < P; p++) {
+      double x[4];
+      for (int s = 0; s < 4; s++) x[s] = pl[p * 4 + s] * pr[p * 4 + s];
+      double lk = dot4(x, freqs);
+      c->scales[snew * P + p] = 0;
+      if (c->scales[sold * P + p]) c->ss[p] -= c->scale_ln;
+      c->lik[p] = log(lk) - (double)c->ss[p];
+    }
+  }
+}
+
+void or_prop4_accept(or_prop4 *c) { /* mod.rs:143-153 */
+  memcpy(c->ss_bak, c->ss, c->P * 4);
+  or_editbuf_accept(c->sel, c->N);
+}
+
+void or_prop4_reject(or_prop4 *c) { /* mod.rs:155-164 */
+  memcpy(c->ss, c->ss_bak, c->P * 4);
+  or_editbuf_reject(c->sel, c->N);
+}
+
+const double *or_prop4_likelihoods(const or_prop4 *c) { return c->lik; }
 
 /* ======================================================================== */
 /* Per-category models with per-edge clock rates, evaluated from scratch.

@@ -145,6 +145,22 @@ void or_gamma_lik_set_alpha(or_gamma_lik *g, double alpha);
 const double *or_gamma_lik_site(const or_gamma_lik *g, int category);
 const double *or_gamma_lik_rates(const or_gamma_lik *g);
 
+/* ---- Propagation calculator (calculator/cuda_propagations) ------------- */
+typedef struct or_prop4 or_prop4;
+/* samples: [L x P] IUPAC bytes (no remapping) */
+or_prop4 *or_prop4_new(size_t num_patterns, const uint8_t *samples,
+                       uint32_t num_leaves, uint32_t scale_ln);
+void or_prop4_free(or_prop4 *c);
+/* DatedTree::propagation_lists: nodes[num_nodes] (updated leaves first, then
+ * internals children first, root last), children[2 (num_nodes -
+ * num_leaves)]; mats: [num_nodes - 1][4][4], node order */
+void or_prop4_propose(or_prop4 *c, uint32_t num_nodes, const uint32_t *nodes,
+                      uint32_t num_leaves, const uint32_t *children,
+                      const double *mats, const double *freqs);
+void or_prop4_accept(or_prop4 *c);
+void or_prop4_reject(or_prop4 *c);
+const double *or_prop4_likelihoods(const or_prop4 *c);
+
 /* ---- Per-category models and per-edge clock rates, from scratch -------- */
 /* HeteroLikelihood's total (hetero.rs:122-160) when classes != NULL, else
  * GammaLikelihood's mix (gamma.rs:126-166).  models[K]; edge_rates[K][N]

@@ -105,6 +105,12 @@ def lib():
         "or_gamma_lik_site": (C.POINTER(C.c_double), [vp, C.c_int]),
         "or_gamma_lik_rates": (C.POINTER(C.c_double), [vp]),
         "or_deduplicate": (C.c_size_t, [C.c_uint32, C.c_size_t, u8p, u8p, u32p, szp]),
+        "or_prop4_new": (vp, [C.c_size_t, u8p, C.c_uint32, C.c_uint32]),
+        "or_prop4_free": (None, [vp]),
+        "or_prop4_propose": (None, [vp, C.c_uint32, u32p, C.c_uint32, u32p, f64p, f64p]),
+        "or_prop4_accept": (None, [vp]),
+        "or_prop4_reject": (None, [vp]),
+        "or_prop4_likelihoods": (C.POINTER(C.c_double), [vp]),
         "or_components_eval": (C.c_double, [vp, C.c_int, vp, f64p, u8p, C.c_size_t, u32p, vp,
                                             C.c_uint32, f64p]),
     }
@@ -344,6 +350,40 @@ class Calc4:
 
     def likelihoods(self):
         p = lib().or_calc4_likelihoods(self._h)
+        return np.ctypeslib.as_array(p, shape=(self.P,)).copy()
+
+
+class Prop4:
+    """CudaPropagationsCalculator's algorithm on the CPU
+    (crates/b3/src/calculator/cuda_propagations/kernels.cu:73-187, mod.rs:77-164)."""
+
+    def __init__(self, samples, num_patterns, scale_ln=30):
+        s = np.ascontiguousarray(samples, np.uint8).reshape(-1)
+        self.P = int(num_patterns)
+        self.L = s.size // self.P
+        self._h = lib().or_prop4_new(self.P, s, self.L, int(scale_ln))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().or_prop4_free(self._h)
+            self._h = None
+
+    def propose(self, nodes, children, num_leaves, mats, freqs):
+        """mats: [len(nodes) - 1][4][4], one per listed node but the root."""
+        nodes = np.ascontiguousarray(nodes, np.uint32)
+        lib().or_prop4_propose(self._h, nodes.size, nodes, int(num_leaves),
+                               np.ascontiguousarray(children, np.uint32).reshape(-1),
+                               np.ascontiguousarray(mats, np.float64).reshape(-1),
+                               np.ascontiguousarray(freqs, np.float64))
+
+    def accept(self):
+        lib().or_prop4_accept(self._h)
+
+    def reject(self):
+        lib().or_prop4_reject(self._h)
+
+    def likelihoods(self):
+        p = lib().or_prop4_likelihoods(self._h)
         return np.ctypeslib.as_array(p, shape=(self.P,)).copy()
 
 

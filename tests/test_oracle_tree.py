@@ -136,4 +136,9 @@ def test_python_mirror_matches_oracle_schedule(seed):
         mi, mc = m.partials_lists()
         np.testing.assert_array_equal(oi, mi)
         np.testing.assert_array_equal(oc, mc)
+        on, ocp, onl = o.propagation_lists()
+        mn, mcp, mnl = m.propagation_lists()
+        np.testing.assert_array_equal(on, mn)
+        np.testing.assert_array_equal(ocp, mcp)
+        assert onl == mnl
         assert o.is_changed() == m.is_changed()
