@@ -8,10 +8,11 @@ from .parameters import ClassVector, Real
 from .mcmc import Chain, Surrogate, ess
 from .substitution import GTR, GY94, HKY, JC, K80, SENSE_CODONS
 from .tree import TOMBSTONE, DatedTree
+from .tuning import autotune
 
 __all__ = [
     "Batch", "Calculator", "HipCalculator", "gamma_rates", "DNALikelihood", "GammaLikelihood",
     "HeteroLikelihood", "Clock", "StrictClock", "RelaxedClock", "ClassVector", "Real",
     "deduplicate", "GTR", "GY94", "HKY", "JC", "K80", "SENSE_CODONS", "TOMBSTONE",
-    "DatedTree", "Chain", "Surrogate", "ess",
+    "DatedTree", "Chain", "Surrogate", "ess", "autotune",
 ]

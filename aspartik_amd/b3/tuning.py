@@ -1,0 +1,120 @@
+"""Measured choice of storage layout and tile shape for one workload.
+
+b3 picks its GPU calculator by configuration (calculator/mod.rs:94-156; the
+propagation calculator is the default "cuda" backend).  Here the choice is
+measured: every candidate handle (partial storage with each valid tile shape
+and patterns per lane, and propagation storage) evaluates the caller's own
+tree, alignment and model -- full evaluations (every edge dirty) and partial
+ones (one random edge and its ancestors, as an MCMC move produces) -- and
+the candidate with the lowest expected time per evaluation under the given
+mix wins.  All candidates compute the same likelihood; only the schedule
+and storage differ.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from .calculator import Calculator
+from .tree import DatedTree
+
+
+@dataclass
+class Trial:
+    name: str
+    config: Calculator
+    full_ms: float
+    partial_ms: float
+    score_ms: float
+    full_kernel_us: float = 0.0
+    partial_kernel_us: float = 0.0
+
+
+def _lists(tree: DatedTree, propagations: bool):
+    if propagations:
+        nodes, children, nl = tree.propagation_lists()
+        flat = nodes[:-1].astype(np.int64)
+        return ("p", nodes, children, nl,
+                tree.heights[tree.parents[flat]] - tree.heights[flat])
+    internals, children = tree.partials_lists()
+    flat = children.reshape(-1).astype(np.int64)
+    return ("i", internals, children, tree.heights[tree.parents[flat]] - tree.heights[flat])
+
+
+def _propose(calc, lst, rates):
+    if lst[0] == "p":
+        calc.propose_propagations_times(lst[1], lst[2], lst[3], lst[4], rates)
+    else:
+        calc.propose_times(lst[1], lst[2], lst[3], rates)
+
+
+def default_candidates(num_categories: int, device: int = 0):
+    """Partial storage with every tile that fits a 1024-thread workgroup and
+    1 or 2 patterns per lane, plus propagation storage."""
+    out = []
+    t = 1
+    while 64 * num_categories * t <= 1024:
+        for ppl in (1, 2):
+            out.append((f"partials T={t} ppl={ppl}",
+                        Calculator.HIP(device, tile_patterns=64 * t, lane_patterns=ppl)))
+        t *= 2
+    out.append(("propagations", Calculator.HIPPropagations(device)))
+    return out
+
+
+def autotune(codes, weights, tree: DatedTree, substitution, category_rates,
+             partial_fraction: float = 0.9, evals: int = 20, candidates=None, seed: int = 0):
+    """Returns (best Calculator, [Trial, ...] sorted by score).
+
+    partial_fraction: share of partial evaluations in the expected mix (an
+    MCMC chain over a fixed alignment is mostly partial updates)."""
+    codes = np.ascontiguousarray(codes, np.uint8)
+    P = codes.shape[1]
+    rates = np.ascontiguousarray(category_rates, np.float64)
+    K = rates.size
+    rng = np.random.default_rng(seed)
+    edges = [int(e) for e in rng.choice(tree.num_nodes, size=16) if int(e) != tree.root]
+    trials = []
+    for name, cfg in candidates or default_candidates(K):
+        calc = cfg.make(codes, P, K)
+        calc.set_weights(weights)
+        calc.set_model(substitution.q_matrix(), substitution.frequencies(),
+                       substitution.special_cases)
+        t = tree.copy()
+        t.mark_all_edges_updated()
+        full = _lists(t, cfg.propagations)
+        t.accept()
+        parts = []
+        for e in edges:
+            t.mark_edge_updated(e)
+            parts.append(_lists(t, cfg.propagations))
+            t.accept()
+        for _ in range(3):
+            _propose(calc, full, rates)
+            calc.total()
+            calc.accept()
+        calc.profile_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(evals):
+            _propose(calc, full, rates)
+            calc.total()
+            calc.accept()
+        full_ms = (time.perf_counter() - t0) / evals * 1e3
+        fk, fn = calc.profile_read()
+        calc.profile_enable(True)
+        t0 = time.perf_counter()
+        for i in range(evals):
+            _propose(calc, parts[i % len(parts)], rates)
+            calc.total()
+            calc.accept()
+        partial_ms = (time.perf_counter() - t0) / evals * 1e3
+        pk, pn = calc.profile_read()
+        calc.profile_enable(False)
+        calc.close()
+        score = (1.0 - partial_fraction) * full_ms + partial_fraction * partial_ms
+        trials.append(Trial(name, cfg, full_ms, partial_ms, score,
+                            fk / max(1, fn) * 1e3, pk / max(1, pn) * 1e3))
+    trials.sort(key=lambda tr: tr.score_ms)
+    return trials[0].config, trials

@@ -16,6 +16,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload codon                     (GY94 61-state, MFMA)
        python bench.py --workload mcmc                      (ESS/s, delayed acceptance)
        python bench.py --workload hetero                    (HeteroLikelihood, relaxed clocks)
+       python bench.py --workload storage                   (partial vs propagation storage)
 """
 from __future__ import annotations
 
@@ -483,6 +484,56 @@ def bench_hetero(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_storage(args):
+    """Partial vs propagation storage on the GTR+Gamma4 workload: the
+    autotuner's table over every partial-storage tile shape and the FP32
+    propagation layout (full evaluations and one-edge partial evaluations,
+    host round trip included), plus b3's two GPU algorithms at FP64
+    (partials = B3H_F64_PLAIN, propagations = B3H_F64_PROPAGATION) for
+    reference.  The expected time per evaluation weights partial updates
+    0.9, as in an MCMC chain."""
+    from aspartik_amd import workloads
+    from aspartik_amd.b3 import Calculator, DatedTree, autotune
+    from aspartik_amd.b3.tuning import default_candidates
+
+    wl = workloads.gtr_gamma4(args.taxa, args.patterns)
+    L, P, K = args.taxa, args.patterns, wl["K"]
+    tree = DatedTree(L, wl["parents"], wl["children"], wl["heights"])
+    evals = max(20, args.steps // 4)
+    cands = default_candidates(K) + [("fp64 partials (b3 cuda_partials)", Calculator.HIPPlainF64()),
+                                     ("fp64 propagations (b3 cuda_propagations)",
+                                      Calculator.HIPPropagationsF64())]
+    log(f"[bench] autotuning {len(cands)} candidates, {evals} evaluations each")
+    best, trials = autotune(wl["codes"], wl["weights"], tree, wl["model"], wl["rates"],
+                            evals=evals, candidates=cands)
+    table = [{"layout": t.name, "full_ms": t.full_ms, "partial_ms": t.partial_ms,
+              "expected_ms": t.score_ms, "full_kernel_us": t.full_kernel_us,
+              "partial_kernel_us": t.partial_kernel_us} for t in trials]
+    for t in table:
+        log(f"[bench] {t['layout']:42s} full {t['full_ms']:.3f} ms ({t['full_kernel_us']:.1f} us)"
+            f"  partial {t['partial_ms']:.3f} ms ({t['partial_kernel_us']:.1f} us)")
+    line = {
+        "metric": "expected logL evals/s at 90% partial updates, best storage layout",
+        "value": 1e3 / trials[0].score_ms,
+        "unit": "logL evals/s",
+        "n_gpus": 1,
+        "steps": evals,
+        "warmup": 3,
+        "ms_per_step": trials[0].score_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Yule tree, sequences simulated along it)",
+        "config": {"workload": f"gtr_gamma4_{L}taxa_{P}pat", "taxa": L, "patterns": P,
+                   "categories": K, "chosen": trials[0].name},
+        "roofline": None,
+        "cpu_baseline": None,
+        "layouts": table,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -493,7 +544,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-f32", action="store_true", help="profiling runs: FP32 kernel only")
-    ap.add_argument("--workload", choices=["gtr128", "batch", "codon", "mcmc", "hetero"],
+    ap.add_argument("--workload", choices=["gtr128", "batch", "codon", "mcmc", "hetero",
+                                           "storage"],
                     default="gtr128")
     ap.add_argument("--mcmc-taxa", type=int, default=200)
     ap.add_argument("--mcmc-sites", type=int, default=200_000)
@@ -519,6 +571,8 @@ def main():
         return bench_mcmc(args)
     if args.workload == "hetero":
         return bench_hetero(args)
+    if args.workload == "storage":
+        return bench_storage(args)
 
     from aspartik_amd import workloads
     from aspartik_amd.b3 import Calculator, DatedTree

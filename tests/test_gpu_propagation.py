@@ -218,3 +218,23 @@ def test_propagation_protocol_errors(hip_device):
         prop.propose_propagations_times(nodes, pch, nl, np.ones(nodes.size - 1), [1.0])
     with pytest.raises(RuntimeError):
         prop.set_category_model(0, pm.q_matrix(), pm.frequencies())
+
+
+def test_autotune_candidates_agree(hip_device):
+    """Every candidate the autotuner times computes the same total (the
+    choice changes the schedule and the storage, not the value)."""
+    from aspartik_amd.b3 import autotune
+    from aspartik_amd.b3.tuning import default_candidates
+    L, P = 30, 3000
+    rng, arrays, om, pm, codes, w = _setup(L, P, seed=21)
+    rates = gamma_rates(0.5, 4)
+    tree = DatedTree(L, *arrays)
+    best, trials = autotune(codes, w, tree, pm, rates, evals=4)
+    assert {t.name for t in trials} == {n for n, _ in default_candidates(4)}
+    assert trials[0].config is best and all(t.full_ms > 0 and t.partial_ms > 0 for t in trials)
+    totals = []
+    for name, cfg in default_candidates(4):
+        lik = GammaLikelihood(codes, w, tree.copy(), pm, num_categories=4, alpha=0.5,
+                              calculator=cfg)
+        totals.append(lik.likelihood())
+    assert max(totals) - min(totals) <= 2e-6 * abs(totals[0])
