@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libb3hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "b3hip.h")
 MCMC_HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "b3mcmc.h")
+DATA_HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "b3data.h")
 
 B3H_OK = 0
 STATUS_NAMES = {
@@ -197,6 +198,22 @@ MCMC_SIGNATURES = {
 }
 
 
+# every symbol declared in include/b3data.h
+DATA_SIGNATURES = {
+    "b3d_msa_from_fasta": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "b3d_msa_from_fasta_file": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "b3d_msa_destroy": (C.c_int, [C.c_void_p]),
+    "b3d_msa_num_sequences": (C.c_uint32, [C.c_void_p]),
+    "b3d_msa_num_sites": (C.c_uint64, [C.c_void_p]),
+    "b3d_msa_name": (C.c_char_p, [C.c_void_p, C.c_uint32]),
+    "b3d_msa_data": (C.POINTER(C.c_uint8), [C.c_void_p]),
+    "b3d_msa_base_frequencies": (C.c_int, [C.c_void_p, f64p]),
+    "b3d_deduplicate": (C.c_int, [u8p, C.c_uint32, C.c_uint64, C.c_int, u8p, C.c_uint64, u32p,
+                                  np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS"),
+                                  C.POINTER(C.c_uint64)]),
+}
+
+
 def lib():
     """Load libb3hip.so; raises if it is not built."""
     global _lib
@@ -207,7 +224,7 @@ def lib():
             f"{LIB_PATH} is not built; run __graft_entry__.build() "
             "(the HIP library is required: there is no CPU fallback)")
     L = C.CDLL(LIB_PATH)
-    for name, (res, args) in {**SIGNATURES, **MCMC_SIGNATURES}.items():
+    for name, (res, args) in {**SIGNATURES, **MCMC_SIGNATURES, **DATA_SIGNATURES}.items():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args

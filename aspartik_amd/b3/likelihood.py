@@ -18,20 +18,12 @@ from .parameters import ClassVector
 from .tree import DatedTree
 
 
-def deduplicate(msa):
+def deduplicate(msa, device=None):
     """likelihood/mod.rs:74-132: unique columns in first-occurrence order,
-    counts as weights, gap (0) -> 0b1111.  msa: [num_seqs, num_sites] u8."""
-    msa = np.ascontiguousarray(msa, np.uint8)
-    cols = np.ascontiguousarray(msa.T)
-    view = cols.view(np.dtype((np.void, cols.shape[1])))[:, 0]
-    _, first, inverse, counts = np.unique(view, return_index=True, return_inverse=True,
-                                          return_counts=True)
-    order = np.argsort(first, kind="stable")
-    rank = np.empty_like(order)
-    rank[order] = np.arange(order.size)
-    leaves = msa[:, first[order]].copy()
-    leaves[leaves == 0] = 0x0F
-    return leaves, counts[order].astype(np.uint32), rank[inverse.reshape(-1)].astype(np.int64)
+    counts as weights, gap (0) -> 0b1111.  msa: [num_seqs, num_sites] u8 or
+    an ``aspartik_amd.data.MSA``; device None: host, an int: that GPU."""
+    from ..data import deduplicate as _dedup
+    return _dedup(msa, device)
 
 
 def _child_times(tree, children):

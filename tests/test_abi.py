@@ -39,6 +39,12 @@ def test_library_exports_every_declared_symbol():
         assert name in exported_m, name
         getattr(lib, name)
     assert set(_lib.MCMC_SIGNATURES) == set(mcmc)
+    data = _declared(_lib.DATA_HEADER_PATH, "b3d")
+    exported_d = set(re.findall(r"\bT (b3d_\w+)", out))
+    for name in data:
+        assert name in exported_d, name
+        getattr(lib, name)
+    assert set(_lib.DATA_SIGNATURES) == set(data)
 
 
 def test_mcmc_struct_layouts_match_the_header():
