@@ -7,4 +7,6 @@ models, DNA and gamma likelihoods).
 """
 from ._lib import B3hError, device_count, lib  # noqa: F401
 
-__all__ = ["B3hError", "device_count", "lib"]
+__version__ = "0.1.0"
+
+__all__ = ["B3hError", "device_count", "lib", "__version__"]

@@ -193,6 +193,9 @@ MCMC_SIGNATURES = {
                                 C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(RunStats)]),
     "b3m_chain_set_surrogate": (C.c_int, [C.c_void_p, C.c_void_p]),
     "b3m_chain_params": (C.c_int, [C.c_void_p, u32p, u32p, f64p, f64p, C.POINTER(C.c_double)]),
+    "b3m_chain_dump_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t,
+                                       C.POINTER(C.c_size_t)]),
+    "b3m_chain_load_state": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "b3m_chain_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]),
 }

@@ -4,7 +4,7 @@ from .batch import Batch
 from .calculator import Calculator, HipCalculator, gamma_rates
 from .clock import Clock, RelaxedClock, StrictClock
 from .likelihood import DNALikelihood, GammaLikelihood, HeteroLikelihood, deduplicate
-from .parameters import ClassVector, Real
+from .parameters import ClassVector, IntVector, Real, RealVector
 from .mcmc import Chain, Surrogate, ess
 from .substitution import GTR, GY94, HKY, JC, K80, SENSE_CODONS
 from .tree import TOMBSTONE, DatedTree
@@ -12,7 +12,7 @@ from .tuning import autotune
 
 __all__ = [
     "Batch", "Calculator", "HipCalculator", "gamma_rates", "DNALikelihood", "GammaLikelihood",
-    "HeteroLikelihood", "Clock", "StrictClock", "RelaxedClock", "ClassVector", "Real",
+    "HeteroLikelihood", "Clock", "StrictClock", "RelaxedClock", "ClassVector", "Real", "RealVector", "IntVector",
     "deduplicate", "GTR", "GY94", "HKY", "JC", "K80", "SENSE_CODONS", "TOMBSTONE",
     "DatedTree", "Chain", "Surrogate", "ess", "autotune",
 ]

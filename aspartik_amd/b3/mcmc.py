@@ -164,6 +164,51 @@ class Chain:
         check(lib().b3m_chain_params(self._h, par, ch, h, r, C.byref(a)))
         return par, ch.reshape(-1, 2), h, r, a.value
 
+    def dump_state(self) -> bytes:
+        """The chain's state in b3's verbatim state format (mcmc.rs:250-268)."""
+        n = C.c_size_t()
+        check(lib().b3m_chain_dump_state(self._h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(lib().b3m_chain_dump_state(self._h, buf, n.value, C.byref(n)))
+        return buf.raw[:n.value]
+
+    def load_state(self, state: bytes):
+        """Mcmc::load_state (mcmc.rs:183-213): parameters, scheduler and
+        generator; the likelihood is evaluated from scratch."""
+        state = bytes(state)
+        check(lib().b3m_chain_load_state(self._h, state, len(state)))
+
+    def run_traced(self, steps, every, writer, tree=None, rates=None, alpha=None,
+                   mode="plain"):
+        """Run ``steps`` steps and hand the state to a ``TraceWriter`` every
+        ``every`` steps (trace_writer.rs:347-369).  ``tree`` (DatedTree),
+        ``rates`` (RealVector) and ``alpha`` (Real) are the mirrors the
+        writer's items reference; they are set from the chain before each
+        row.  Returns the concatenated run statistics of the chunks."""
+        from .parameters import Real, RealVector
+        from .tree import DatedTree
+        done, stats = 0, []
+        while done < steps:
+            n = min(every, steps - done)
+            _, st, _ = self.run(n, mode=mode, record_every=n)
+            stats.append(st)
+            done += n
+            par, ch, h, r, a = self.params()
+            if isinstance(tree, DatedTree):
+                tree.load_parts(par, h)
+                tree.accept()
+            if isinstance(rates, RealVector):
+                for j in range(6):
+                    rates[j] = r[j]
+                rates.accept()
+            if isinstance(alpha, Real):
+                alpha.set(a)
+                alpha.accept()
+            ll, lp, post = self.state()
+            life = int(np.frombuffer(self.dump_state()[8:16], "<u8")[0])
+            writer.call(life, post, lp, ll)
+        return stats
+
     def set_surrogate(self, s: Surrogate | None):
         self.surrogate = s
         check(lib().b3m_chain_set_surrogate(self._h, None if s is None else s._h))

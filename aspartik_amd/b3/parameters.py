@@ -1,4 +1,4 @@
-"""Parameters the clocks and HeteroLikelihood read: ``Real`` and ``ClassVector``.
+"""Parameters: ``Real``, ``RealVector``, ``IntVector`` and ``ClassVector``.
 
 They mirror b3's Parameter protocol (crates/b3/src/parameters/mod.rs):
 ``is_changed`` since the last ``accept``/``reject``, ``accept`` keeps the
@@ -45,6 +45,61 @@ class Real:
 
     def __repr__(self):
         return f"Real({self._value})"
+
+
+class _Vector:
+    _dtype = np.float64
+
+    def __init__(self, *values):
+        if len(values) == 1 and np.ndim(values[0]) == 1:
+            values = tuple(values[0])
+        if not values:
+            raise ValueError("a vector parameter must be non-empty")
+        self._values = np.array(values, self._dtype)
+        self._accepted = self._values.copy()
+        self._indices: list[int] = []
+
+    @classmethod
+    def repeat(cls, value, length: int):
+        return cls(*([value] * int(length)))
+
+    def __len__(self) -> int:
+        return self._values.size
+
+    def __getitem__(self, index: int):
+        return self._values[index].item()
+
+    def __setitem__(self, index: int, value):
+        self._values[index] = value
+        self._indices.append(int(index))
+
+    @property
+    def values(self) -> np.ndarray:
+        return self._values.copy()
+
+    def changed_indices(self) -> list[int]:
+        return list(self._indices)
+
+    def is_changed(self) -> bool:
+        return bool(self._indices)
+
+    def accept(self) -> None:
+        self._accepted[:] = self._values
+        self._indices.clear()
+
+    def reject(self) -> None:
+        self._values[:] = self._accepted
+        self._indices.clear()
+
+
+class RealVector(_Vector):
+    """parameters/real_vector.rs"""
+    _dtype = np.float64
+
+
+class IntVector(_Vector):
+    """parameters/int_vector.rs"""
+    _dtype = np.int64
 
 
 class ClassVector:

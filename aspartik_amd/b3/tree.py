@@ -140,6 +140,42 @@ class DatedTree:
             raise ValueError(f"replace_child: {old_child} is not a child of {parent}")
         self.mark_edge_updated(new_child)
 
+    def load_parts(self, parents, heights):
+        """DatedTree::load_parts (tree.rs:1135-1151, load_parents 906-930):
+        parents [N] (0xFFFFFFFF for the root) and heights [N]; children are
+        rebuilt in node order.  b3 leaves the dirty marks alone here; this
+        mirror marks every edge so that the next evaluation sees the new
+        tree (as the verbatim load path does, tree.rs:997-1009)."""
+        parents = np.asarray(parents).astype(np.int64)
+        heights = np.asarray(heights, np.float64)
+        if parents.size != self.N or heights.size != self.N:
+            raise ValueError("parents and heights must have one entry per node")
+        parents[parents == TOMBSTONE] = -1
+        children = np.full((self.L - 1, 2), -1, np.int64)
+        root = None
+        for node in range(self.N):
+            par = int(parents[node])
+            if par < 0:
+                root = node
+                continue
+            row = children[par - self.L]
+            row[0 if row[0] < 0 else 1] = node
+        if root is None or (children < 0).any():
+            raise ValueError("parents do not describe a binary tree")
+        self.parents = parents
+        self.children = children
+        self.heights = heights.copy()
+        self.root = int(root)
+        self.mark_all_edges_updated()
+
+    def total_length(self) -> float:
+        """tree.rs:580-582: edge lengths summed in edge (node) order."""
+        total = 0.0
+        for e in range(self.N):
+            if e != self.root:
+                total += float(self.heights[self.parents[e]] - self.heights[e])
+        return total
+
     def set_root(self, node):
         self.root = int(node)
         self.parents[node] = -1

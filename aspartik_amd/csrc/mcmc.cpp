@@ -399,6 +399,80 @@ void tune(b3m_chain *c) {
   }
 }
 
+// b3's verbatim format (crates/verbatim/src/lib.rs): little-endian
+// primitives, each padded to a multiple of its size from the stream start;
+// tags unpadded; a slice is a u32 length, padding to the element alignment
+// when non-empty, then the elements.
+struct VWriter {
+  std::vector<uint8_t> out;
+  void pad(size_t a) {
+    while (out.size() % a) out.push_back(0);
+  }
+  template <typename T>
+  void prim(T v) {
+    pad(sizeof(T));
+    const uint8_t *b = reinterpret_cast<const uint8_t *>(&v);
+    out.insert(out.end(), b, b + sizeof(T));
+  }
+  void tag(const uint8_t *t, size_t n) { out.insert(out.end(), t, t + n); }
+  template <typename T>
+  void slice(const T *v, uint32_t n) {
+    prim<uint32_t>(n);
+    if (n) pad(alignof(T));
+    const uint8_t *b = reinterpret_cast<const uint8_t *>(v);
+    out.insert(out.end(), b, b + (size_t)n * sizeof(T));
+  }
+};
+
+struct VReader {
+  const uint8_t *p;
+  size_t len, pos = 0;
+  bool ok = true;
+  bool skip_pad(size_t a) {
+    const size_t r = pos % a;
+    if (r) pos += a - r;
+    return pos <= len;
+  }
+  template <typename T>
+  T prim() {
+    T v{};
+    if (!ok || !skip_pad(sizeof(T)) || pos + sizeof(T) > len) {
+      ok = false;
+      return v;
+    }
+    std::memcpy(&v, p + pos, sizeof(T));
+    pos += sizeof(T);
+    return v;
+  }
+  bool tag(const uint8_t *t, size_t n) {
+    if (!ok || pos + n > len || std::memcmp(p + pos, t, n) != 0) return ok = false;
+    pos += n;
+    return true;
+  }
+  template <typename T>
+  std::vector<T> slice() {
+    const uint32_t n = prim<uint32_t>();
+    std::vector<T> v;
+    if (!ok) return v;
+    if (n && !skip_pad(alignof(T))) {
+      ok = false;
+      return v;
+    }
+    if (pos + (size_t)n * sizeof(T) > len) {
+      ok = false;
+      return v;
+    }
+    v.resize(n);
+    std::memcpy(v.data(), p + pos, (size_t)n * sizeof(T));
+    pos += (size_t)n * sizeof(T);
+    return v;
+  }
+};
+
+// Mcmc's state header (mcmc.rs:19, lib.rs:29)
+constexpr uint8_t kStateTag[4] = {0x62, 0x33, 0xb3, 0x44};
+constexpr uint32_t kStateVersion = 19;
+
 double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
       .count();
@@ -542,6 +616,112 @@ b3h_status b3m_chain_state(const b3m_chain *c, double *loglik, double *logprior,
   if (loglik) *loglik = c->loglik;
   if (logprior) *logprior = c->logprior;
   if (posterior) *posterior = c->posterior;
+  return B3H_OK;
+}
+
+b3h_status b3m_chain_dump_state(const b3m_chain *c, uint8_t *buf, size_t cap, size_t *len) {
+  if (!c || !len) return fail(B3H_ERR_ARG, "null argument");
+  VWriter w;
+  w.tag(kStateTag, 4);
+  w.prim<uint32_t>(kStateVersion);
+  w.prim<uint64_t>(c->life_steps);
+  w.prim<double>(c->posterior);
+  // parameters: the tree (tree.rs:1011-1016), the exchangeabilities as a
+  // RealVector (real_vector.rs:77-82), alpha as a Real (real.rs:59-62)
+  w.slice(c->tree.parents.data(), (uint32_t)c->tree.parents.size());
+  w.slice(c->tree.heights.data(), (uint32_t)c->tree.heights.size());
+  for (int i = 0; i < 6; i++) w.prim<double>(c->rates[i]);
+  w.prim<double>(c->alpha);
+  // scheduler (operators/mod.rs:195-217,488-507): accepts, rejects since the
+  // last tune, and the tuning of operators that have one
+  const double tuning[kNumOps] = {-1.0, c->tree_scale_f, c->delta_t, c->alpha_f, -1.0};
+  for (int op = 0; op < kNumOps; op++) {
+    w.prim<uint32_t>((uint32_t)c->tune_acc[op]);
+    w.prim<uint32_t>((uint32_t)c->tune_rej[op]);
+    if (tuning[op] >= 0.0) w.prim<double>(tuning[op]);
+  }
+  // the generator as two u128 words (rng/src/lib.rs:46-51 writes PCG64's
+  // state and increment; this chain's xoshiro256** state fills the same
+  // 32 bytes)
+  w.prim<uint64_t>(c->rng.s[0]);
+  w.prim<uint64_t>(c->rng.s[1]);
+  w.prim<uint64_t>(c->rng.s[2]);
+  w.prim<uint64_t>(c->rng.s[3]);
+  *len = w.out.size();
+  if (!buf) return B3H_OK;
+  if (cap < w.out.size()) return fail(B3H_ERR_ARG, "state buffer too small");
+  std::memcpy(buf, w.out.data(), w.out.size());
+  return B3H_OK;
+}
+
+b3h_status b3m_chain_load_state(b3m_chain *c, const uint8_t *buf, size_t len) {
+  if (!c || !buf) return fail(B3H_ERR_ARG, "null argument");
+  VReader r{buf, len};
+  if (!r.tag(kStateTag, 4)) return fail(B3H_ERR_ARG, "Not a b3 state file");
+  const uint32_t version = r.prim<uint32_t>();
+  if (r.ok && version != kStateVersion)
+    return fail(B3H_ERR_ARG, "Cannot load state: protocol version v" + std::to_string(version) +
+                                 " is incompatible with the supported version v" +
+                                 std::to_string(kStateVersion));
+  const uint64_t step = r.prim<uint64_t>();
+  (void)r.prim<double>();  // the posterior is recomputed from the parameters
+  std::vector<uint32_t> parents = r.slice<uint32_t>();
+  std::vector<double> heights = r.slice<double>();
+  double rates[6], alpha;
+  for (double &x : rates) x = r.prim<double>();
+  alpha = r.prim<double>();
+  uint32_t acc[kNumOps], rej[kNumOps];
+  double tuning[kNumOps] = {-1.0, 0.0, 0.0, 0.0, -1.0};
+  for (int op = 0; op < kNumOps; op++) {
+    acc[op] = r.prim<uint32_t>();
+    rej[op] = r.prim<uint32_t>();
+    if (tuning[op] >= 0.0) tuning[op] = r.prim<double>();
+  }
+  uint64_t rs[4];
+  for (uint64_t &x : rs) x = r.prim<uint64_t>();
+  if (!r.ok) return fail(B3H_ERR_ARG, "truncated or malformed state");
+  const uint32_t N = 2 * c->L - 1;
+  if (parents.size() != N || heights.size() != N)
+    return fail(B3H_ERR_ARG, "the state's tree has a different number of nodes");
+  // DatedTree::load_parents (tree.rs:906-930): children in node order
+  std::vector<uint32_t> children(2 * (size_t)(c->L - 1), kTomb);
+  for (uint32_t v = 0; v < N; v++) {
+    const uint32_t p = parents[v];
+    if (p == kTomb) continue;
+    if (p < c->L || p >= N) return fail(B3H_ERR_ARG, "parent out of range in the state");
+    uint32_t *slot = &children[2 * (size_t)(p - c->L)];
+    if (slot[0] == kTomb) slot[0] = v;
+    else if (slot[1] == kTomb) slot[1] = v;
+    else return fail(B3H_ERR_ARG, "a node with more than two children in the state");
+  }
+  Tree t;
+  std::string err;
+  if (!t.init(c->L, parents.data(), children.data(), heights.data(), &err))
+    return fail(B3H_ERR_ARG, err);
+  for (double x : rates)
+    if (!(x > 0.0)) return fail(B3H_ERR_ARG, "non-positive rate in the state");
+  if (!(alpha > 0.0)) return fail(B3H_ERR_ARG, "non-positive alpha in the state");
+  // commit, then evaluate from scratch and accept (mcmc.rs:196-213)
+  c->tree = t;
+  std::memcpy(c->rates, rates, sizeof rates);
+  std::memcpy(c->rates_acc, rates, sizeof rates);
+  c->alpha = c->alpha_acc = alpha;
+  c->life_steps = step;
+  for (int op = 0; op < kNumOps; op++) {
+    c->tune_acc[op] = acc[op];
+    c->tune_rej[op] = rej[op];
+  }
+  c->tree_scale_f = tuning[kTreeScale];
+  c->delta_t = tuning[kRates];
+  c->alpha_f = tuning[kAlpha];
+  std::memcpy(c->rng.s, rs, sizeof rs);
+  c->surrogate_valid = false;
+  c->tree.mark_all();
+  c->logprior = log_prior(c);
+  b3h_status st = evaluate(c, &c->loglik);
+  if (st) return st;
+  c->posterior = c->loglik + c->logprior;
+  finalize(c, true);
   return B3H_OK;
 }
 

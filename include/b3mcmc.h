@@ -158,6 +158,22 @@ b3h_status b3m_chain_set_surrogate(b3m_chain *c, b3m_mlp *m);
 b3h_status b3m_chain_params(const b3m_chain *c, uint32_t *parents, uint32_t *children,
                             double *heights, double *gtr_rates, double *alpha);
 
+/* Mcmc::dump / load_state (mcmc.rs:183-268) in b3's verbatim format
+ * (crates/verbatim/src/lib.rs): tag 62 33 b3 44, protocol version 19, the
+ * step count (u64) and the posterior (f64); then the parameters in order --
+ * the tree as DatedTree::dump writes it (a u32 slice of parents, an f64
+ * slice of heights), the six exchangeabilities as a RealVector, alpha as a
+ * Real; then the scheduler, per operator (node slide, tree scale, rates,
+ * alpha, narrow exchange) accepts and rejects since the last tune (u32 each)
+ * and the tuning of the three tunable ones (f64); then the generator state
+ * in 32 bytes where b3 writes PCG64's two u128 words.
+ * dump: with buf null only *len is set; otherwise cap must hold *len bytes.
+ * load: replaces the chain's state, rebuilds children from parents as
+ * DatedTree::load_parents does, evaluates the likelihood from scratch and
+ * accepts (the stored posterior is not trusted). */
+b3h_status b3m_chain_dump_state(const b3m_chain *c, uint8_t *buf, size_t cap, size_t *len);
+b3h_status b3m_chain_load_state(b3m_chain *c, const uint8_t *buf, size_t len);
+
 /* current log likelihood / prior / posterior */
 b3h_status b3m_chain_state(const b3m_chain *c, double *loglik, double *logprior,
                            double *posterior);
