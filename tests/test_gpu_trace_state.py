@@ -56,8 +56,14 @@ def test_chain_resume_from_state_is_exact(hip_device):
     np.testing.assert_array_equal(trace_a[:, 1:], trace_b[:, 1:])
     assert stats_a["op_accepted"] == stats_b["op_accepted"]
     assert stats_a["tuning"] == stats_b["tuning"]
-    for x, y in zip(a.params(), b.params()):
-        np.testing.assert_array_equal(x, y)
+    pa_, ca, ha, ra, aa = a.params()
+    pb, cb, hb, rb, ab = b.params()
+    np.testing.assert_array_equal(pa_, pb)
+    np.testing.assert_array_equal(ha, hb)
+    np.testing.assert_array_equal(ra, rb)
+    assert aa == ab
+    # the loaded tree's children were rebuilt in node order (tree.rs:913-927)
+    np.testing.assert_array_equal(np.sort(ca, axis=1), np.sort(cb, axis=1))
     # the Python reader of the verbatim format decodes the native file
     tree = DatedTree(d["L"], d["par"], d["ch"], d["h"])
     rates, alpha = RealVector.repeat(0.0, 6), Real(0.0)
