@@ -5,87 +5,142 @@
 // 66-101, called from substitution.rs:544-554) and the per-edge host build
 // (transitions.rs:26-48, substitution.rs:403-420,571-588).  Q is reversible
 // for every model b3 exports, so S = D^1/2 Q D^-1/2 (D = diag(pi)) is
-// symmetric and is diagonalised by cyclic Jacobi rotations in one wavefront.
+// symmetric and is diagonalised by parallel cyclic Jacobi rotations.
 #include "common.hpp"
 #include "kernels.hpp"
 
 namespace b3h {
 
-// One wavefront, n <= 64.  A and U live in LDS (2 x 32 KiB).  Lane k owns
-// row/column k of every rotation, so each rotation costs a handful of LDS
-// operations; rotations of a sweep are applied in the cyclic (p, q) order.
-__global__ __launch_bounds__(64) void eigen_kernel(DevModel *m) {
-  __shared__ double A[64 * 64];
-  __shared__ double U[64 * 64];
+// Parallel cyclic Jacobi, n <= 64, one 1024-thread workgroup; A and U in
+// LDS (2 x 32 KiB).  Indices are paired by the round-robin ("chess
+// tournament") ordering: m = n rounded up to even, m - 1 rounds per sweep,
+// m / 2 disjoint (p, q) pairs per round.  Disjoint rotations commute, so a
+// round applies all its column rotations, then all its row rotations
+// (A <- J^T A J, U <- U J), with one barrier between phases.  A pair with a
+// padding index or a negligible a_pq is the identity.
+__global__ __launch_bounds__(1024) void eigen_kernel(DevModel *m) {
+  // rows padded to 65 doubles: a column walk over k spreads over the banks
+  constexpr int LD = 65;
+  __shared__ double A[64 * LD];
+  __shared__ double U[64 * LD];
+  __shared__ double rc[32], rs[32];
+  __shared__ int rp[32], rq[32];
+  __shared__ double red_off[16], red_tot[16];
+  __shared__ int rot_any;
   const int n = m->n;
-  const int k = threadIdx.x;
-  if (k < n) {
-    const double sk = sqrt(m->freqs[k]);
-    for (int j = 0; j < n; j++) {
-      const double sj = sqrt(m->freqs[j]);
+  const int tid = threadIdx.x;
+  const int mm = (n + 1) & ~1;
+  const int npairs = mm / 2;
+  for (int idx = tid; idx < 64 * 64; idx += blockDim.x) {
+    const int k = idx >> 6, j = idx & 63;
+    double a = 0.0;
+    if (k < n && j < n) {
+      const double sk = sqrt(m->freqs[k]), sj = sqrt(m->freqs[j]);
       const double skj = m->q[k * n + j] * sk / sj;
       const double sjk = m->q[j * n + k] * sj / sk;
-      A[k * 64 + j] = 0.5 * (skj + sjk);
-      U[k * 64 + j] = (k == j) ? 1.0 : 0.0;
+      a = 0.5 * (skj + sjk);
     }
+    A[k * LD + j] = a;
+    U[k * LD + j] = (k == j) ? 1.0 : 0.0;
   }
   __syncthreads();
   for (int sweep = 0; sweep < 64; sweep++) {
-    // convergence test: off-diagonal mass relative to total
+    // convergence test: off-diagonal mass relative to the total
     double off = 0.0, tot = 0.0;
-    if (k < n)
-      for (int j = 0; j < n; j++) {
-        const double v = A[k * 64 + j] * A[k * 64 + j];
+    for (int idx = tid; idx < n * 64; idx += blockDim.x) {
+      const int k = idx >> 6, j = idx & 63;
+      if (j < n) {
+        const double v = A[k * LD + j] * A[k * LD + j];
         tot += v;
         if (j != k) off += v;
       }
-    for (int s = 32; s > 0; s >>= 1) {
-      off += __shfl_xor(off, s);
-      tot += __shfl_xor(tot, s);
+    }
+    for (int sh = 32; sh > 0; sh >>= 1) {
+      off += __shfl_xor(off, sh);
+      tot += __shfl_xor(tot, sh);
+    }
+    if ((tid & 63) == 0) {
+      red_off[tid >> 6] = off;
+      red_tot[tid >> 6] = tot;
+    }
+    if (tid == 0) rot_any = 0;
+    __syncthreads();
+    off = 0.0;
+    tot = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+      off += red_off[w];
+      tot += red_tot[w];
     }
     if (off <= 1e-30 * tot || off == 0.0) break;
-    int rotated = 0;
-    for (int p = 0; p < n - 1; p++) {
-      for (int q = p + 1; q < n; q++) {
-        const double apq = A[p * 64 + q];
-        const double app = A[p * 64 + p], aqq = A[q * 64 + q];
-        // uniform: every lane read the same values.  Rotations too small to
-        // change the diagonal at double precision are skipped.
-        if (fabs(apq) <= 1e-18 * (fabs(app) + fabs(aqq)) || apq == 0.0) continue;
-        rotated = 1;
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) /
-                         (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-        __syncthreads();
-        if (k < n) {  // columns p, q
-          const double akp = A[k * 64 + p], akq = A[k * 64 + q];
-          A[k * 64 + p] = c * akp - s * akq;
-          A[k * 64 + q] = s * akp + c * akq;
+    for (int round = 0; round < mm - 1; round++) {
+      if (tid < npairs) {
+        // positions tid and mm - 1 - tid; position 0 is fixed, the others rotate
+        auto player = [&](int pos) { return pos == 0 ? 0 : 1 + (pos - 1 + round) % (mm - 1); };
+        int p = player(tid), q = player(mm - 1 - tid);
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
         }
-        __syncthreads();
-        if (k < n) {  // rows p, q
-          const double apk = A[p * 64 + k], aqk = A[q * 64 + k];
-          A[p * 64 + k] = c * apk - s * aqk;
-          A[q * 64 + k] = s * apk + c * aqk;
-          const double ukp = U[k * 64 + p], ukq = U[k * 64 + q];
-          U[k * 64 + p] = c * ukp - s * ukq;
-          U[k * 64 + q] = s * ukp + c * ukq;
+        double c = 1.0, sn = 0.0;
+        if (q < n) {
+          const double apq = A[p * LD + q];
+          const double app = A[p * LD + p], aqq = A[q * LD + q];
+          if (!(fabs(apq) <= 1e-18 * (fabs(app) + fabs(aqq)) || apq == 0.0)) {
+            const double theta = (aqq - app) / (2.0 * apq);
+            const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            sn = t * c;
+            rot_any = 1;
+          }
         }
-        __syncthreads();
+        rp[tid] = p;
+        rq[tid] = q < n ? q : p;  // identity pairs touch only p
+        rc[tid] = c;
+        rs[tid] = sn;
       }
+      __syncthreads();
+      // columns p, q of A and U: lane k of a 64-lane row walks row k
+      for (int idx = tid; idx < npairs * 64; idx += blockDim.x) {
+        const int i = idx >> 6, k = idx & 63;
+        const double sn = rs[i];
+        if (sn == 0.0) continue;
+        const double c = rc[i];
+        const int p = rp[i], q = rq[i];
+        const double akp = A[k * LD + p], akq = A[k * LD + q];
+        A[k * LD + p] = c * akp - sn * akq;
+        A[k * LD + q] = sn * akp + c * akq;
+        const double ukp = U[k * LD + p], ukq = U[k * LD + q];
+        U[k * LD + p] = c * ukp - sn * ukq;
+        U[k * LD + q] = sn * ukp + c * ukq;
+      }
+      __syncthreads();
+      // rows p, q of A
+      for (int idx = tid; idx < npairs * 64; idx += blockDim.x) {
+        const int i = idx >> 6, k = idx & 63;
+        const double sn = rs[i];
+        if (sn == 0.0) continue;
+        const double c = rc[i];
+        const int p = rp[i], q = rq[i];
+        const double apk = A[p * LD + k], aqk = A[q * LD + k];
+        A[p * LD + k] = c * apk - sn * aqk;
+        A[q * LD + k] = sn * apk + c * aqk;
+      }
+      __syncthreads();
     }
-    if (!rotated) break;
+    // every thread reads the flag before thread 0 may clear it for the next sweep
+    const int any = rot_any;
+    __syncthreads();
+    if (!any) break;
   }
   __syncthreads();
-  if (k < n) {
+  for (int idx = tid; idx < n * n; idx += blockDim.x) {
+    const int k = idx / n, j = idx % n;
     const double sk = sqrt(m->freqs[k]);
-    for (int j = 0; j < n; j++) {
-      m->evec[k * n + j] = U[k * 64 + j] / sk;   // V = D^-1/2 U
-      m->ievc[j * n + k] = U[k * 64 + j] * sk;   // V^-1 = U^T D^1/2
-    }
-    m->eval[k] = A[k * 64 + k];
+    m->evec[k * n + j] = U[k * LD + j] / sk;  // V = D^-1/2 U
+    m->ievc[j * n + k] = U[k * LD + j] * sk;  // V^-1 = U^T D^1/2
   }
+  if (tid < n) m->eval[tid] = A[tid * LD + tid];
 }
 
 // P(t) element (r, col) for an S-state model: the special cases of
@@ -211,7 +266,7 @@ template hipError_t launch_pbuild_prop<double>(uint32_t, uint32_t, int, const do
                                                double *, hipStream_t);
 
 hipError_t launch_eigen(DevModel *m, hipStream_t s) {
-  hipLaunchKernelGGL(eigen_kernel, dim3(1), dim3(64), 0, s, m);
+  hipLaunchKernelGGL(eigen_kernel, dim3(1), dim3(1024), 0, s, m);
   return hipGetLastError();
 }
 
