@@ -11,6 +11,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "reduce.hpp"
 #include "schedule.hpp"
 
 using namespace b3h;
@@ -25,6 +26,12 @@ b3h_status fail(b3h_status s, const std::string &msg) {
 }
 
 }  // namespace
+
+// Route eligible FP32 partial-update lists to the path kernel (1) or keep
+// them on the fused pruning kernel (0).
+#ifndef B3H_PATH_KERNEL
+#define B3H_PATH_KERNEL 1
+#endif
 
 namespace b3h {
 // shared with the other host translation units (batch.hip)
@@ -87,6 +94,7 @@ struct b3h_calc {
   std::vector<uint8_t> editbuf;  // crates/sk/src/editbuf.rs
   std::vector<uint8_t> editbuf_next;
   Scheduler sched;
+  std::vector<uint32_t> path_srcs;  // sources of a path-kernel list, in use order
   double freqs[64] = {0};
   // per-category models (hetero.rs:122-135): model[c] and its frequencies
   bool per_category = false;
@@ -237,7 +245,39 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
   const bool reorder = f32 || codon;
   const uint32_t cap = reorder ? c->sched.capacity : 0;
   std::string err;
-  if (!c->sched.build(n, internals, children, c->L, c->editbuf_next.data(), reorder, cap, &err))
+  // Partial updates on the FP32 path: a list that forms one chain (entry 0
+  // reads two stored children, every later entry the entry before it and
+  // one stored child that no earlier entry of this pass wrote) runs in the
+  // path kernel, which reads its stored children a segment at a time.
+  bool path = false;
+  std::vector<uint32_t> &psrc = c->path_srcs;
+  psrc.clear();
+  if (B3H_PATH_KERNEL && f32 && n <= 64) {
+    if (!c->sched.build(n, internals, children, c->L, c->editbuf_next.data(), true, 0, &err))
+      return fail(B3H_ERR_ARG, err);
+    path = true;
+    for (uint32_t i = 0; i < n && path; i++) {
+      const Entry &e = c->sched.entries[i];
+      const uint32_t l = e.left & SRC_TAG, r = e.right & SRC_TAG;
+      if (l == SRC_LDS || r == SRC_LDS) {
+        path = false;
+      } else if (i == 0) {
+        if (l == SRC_PREV || r == SRC_PREV) path = false;
+        psrc.push_back(e.left);
+        psrc.push_back(e.right);
+      } else if ((l == SRC_PREV) == (r == SRC_PREV)) {
+        path = false;  // a second chain starts here
+      } else {
+        psrc.push_back(l == SRC_PREV ? e.right : e.left);
+      }
+    }
+    for (uint32_t k = 0; k < psrc.size() && path; k++)
+      if ((psrc[k] & SRC_TAG) == SRC_HBM)
+        for (uint32_t j = 0; j + 1 < k && j < n; j++)  // written earlier in this pass
+          if (c->sched.entries[j].out == (psrc[k] & SRC_IDX)) path = false;
+  }
+  if (!path &&
+      !c->sched.build(n, internals, children, c->L, c->editbuf_next.data(), reorder, cap, &err))
     return fail(B3H_ERR_ARG, err);
   c->editbuf.swap(c->editbuf_next);
 
@@ -253,7 +293,8 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
   const size_t off_t = off_m + ((mbytes + 255) & ~(size_t)255);
   const size_t off_r = off_t + ((tbytes + 255) & ~(size_t)255);
   const size_t off_pi = off_r + ((rbytes + 255) & ~(size_t)255);
-  const size_t total = off_pi + 64 * 8;
+  const size_t off_src = off_pi + 64 * 8;
+  const size_t total = off_src + (path ? psrc.size() * 4 : 0);
   if (total > c->stage_bytes) return fail(B3H_ERR_ARG, "staging overflow");
   int s = 0;
   unsigned char *h = next_stage(c, &s);
@@ -292,6 +333,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
   if (pik)
     for (uint32_t k = 0; k < K; k++)
       for (uint32_t j = 0; j < 4; j++) hpi[4 * k + j] = c->freqs_k[k][j];
+  if (path && !psrc.empty()) std::memcpy(h + off_src, psrc.data(), psrc.size() * 4);
   unsigned char *d = c->d_stage[s];
   HIPCHK(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipEventRecord(c->stage_ev[s], c->stream));
@@ -395,11 +437,16 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     // that prefetches them, with one pattern per lane (its registers do not
     // fit two); lists computed from registers and LDS (full evaluations) use
     // the configured patterns per lane.
-    const bool hbm = c->sched.hbm_sources * 8 > n;
-    const uint32_t ppl = hbm ? 1u : c->PPL;
-    const uint32_t tp = 64u * c->T * ppl;
-    a.ntiles = (c->P + tp - 1) / tp;
-    HIPCHK(launch_prune_f32(a, ppl, hbm, (c->P + tp - 1) / tp, c->stream));  // hbm => ppl 1
+    if (path) {
+      a.ntiles = path_tiles(c->P, K);
+      HIPCHK(launch_prune_path(a, reinterpret_cast<const uint32_t *>(d + off_src), c->stream));
+    } else {
+      const bool hbm = c->sched.hbm_sources * 8 > n;
+      const uint32_t ppl = hbm ? 1u : c->PPL;
+      const uint32_t tp = 64u * c->T * ppl;
+      a.ntiles = (c->P + tp - 1) / tp;
+      HIPCHK(launch_prune_f32(a, ppl, hbm, (c->P + tp - 1) / tp, c->stream));  // hbm => ppl 1
+    }
     if (e1) HIPCHK(hipEventRecord(e1, c->stream));
   } else {
     HIPCHK(launch_pbuild<double>(d_entries, n, K, S, mode, d_mats, d_times,
@@ -745,7 +792,12 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   if ((st = dev_alloc(c, (void **)&c->d_mixed, (size_t)2 * P * 8))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_classes, (size_t)P))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_weights, (size_t)P * 4))) return bail(st);
-  if ((st = dev_alloc(c, (void **)&c->d_blocksum, (size_t)std::max<uint32_t>(c->blocks, 1u) * 8)))
+  // the path kernel (partial updates) uses one block per 64 patterns
+  // per-tile partial sums (of the pruning kernel or the path kernel)
+  if ((st = dev_alloc(c, (void **)&c->d_blocksum,
+                      (size_t)std::max<uint32_t>(std::max<uint32_t>(c->blocks, 1u),
+                                                 path_tiles(P, K)) *
+                          8)))
     return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_mix_blocksum, (size_t)mix_blocks(P) * 8))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_counter, 64))) return bail(st);
@@ -761,7 +813,7 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   c->stage_bytes = ((size_t)2 * c->L * sizeof(Entry) + 255) / 256 * 256 +
                    ((size_t)K * 2 * c->I * host_mat + 255) / 256 * 256 +
                    ((size_t)K * 2 * c->I * 8 + 255) / 256 * 256 + ((size_t)K * 8 + 255) / 256 * 256 +
-                   64 * 8 + 256;
+                   64 * 8 + 512;
   for (int s = 0; s < kStages; s++) {
     if (hipHostMalloc((void **)&c->h_stage[s], c->stage_bytes, hipHostMallocDefault) != hipSuccess) {
       fail(B3H_ERR_OOM, "hipHostMalloc staging failed");

@@ -44,8 +44,8 @@ struct F32Args {
   double *site;           // [K][P]
   double *mixed;          // [P]
   const uint32_t *weights;
-  double *blocksum;       // [gridDim.x]
-  uint32_t *counter;      // last-block ticket (self-resetting)
+  double *blocksum;       // [ntiles] per-tile partial sums
+  uint32_t *counter;      // last-tile ticket (self-resetting)
   double *total;
   double pi0, pi1, pi2, pi3;
   const double *pik;        // [K][4] root frequencies per category, or null (pi0..pi3)
@@ -66,6 +66,13 @@ hipError_t launch_prune_f32_batch(const F32Args *insts, const uint2 *items, uint
                                   uint32_t *queue, uint32_t ppl, bool hbm, uint32_t threads,
                                   size_t lds, uint32_t grid, hipStream_t s);
 hipError_t prepare_prune_f32(int max_lds_bytes);
+// Partial updates that form one chain (entry 0 reads two stored children,
+// entry i > 0 the entry before it and one stored child): srcs[n + 1] lists
+// the stored children (SRC_HBM slot or SRC_TIP leaf) in the order the entries
+// read them; one 64-pattern workgroup of K wavefronts.
+// a.ntiles must be path_tiles(P, K)
+hipError_t launch_prune_path(const F32Args &a, const uint32_t *srcs, hipStream_t s);
+uint32_t path_tiles(uint32_t P, uint32_t K);
 
 // ---- prune_codon.hip ----------------------------------------------------
 // S <= 64 states on MFMA: FP16 partials [slot][K][Ppad/32][64 lanes][32],
@@ -146,8 +153,8 @@ struct MixArgs {
   const uint8_t *classes;  // [P] or null
   double *mixed;       // [P]
   const uint32_t *weights;
-  double *blocksum;
-  uint32_t *counter;
+  double *blocksum;    // [mix_blocks(P)]
+  uint32_t *counter;   // last-block ticket (self-resetting)
   double *total;
   double ln_inv_k;
   uint32_t P, K;

@@ -450,7 +450,18 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
 template <int PPL, bool HBM>
 __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
   extern __shared__ __align__(16) unsigned char smem[];
+#if defined(__HIP_DEVICE_COMPILE__)
+  // The record is read where the dispatch put it (offset 0 of the kernel
+  // argument segment) through the constant address space.  Through the
+  // by-value parameter the compiler keeps a private copy and merges the two
+  // ways of reading the root frequencies into one flat load from it.
+  (void)a;
+  typedef const __attribute__((address_space(4))) F32Args ConstArgs;
+  ConstArgs &ka = *(ConstArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+  prune_tile<PPL, HBM>(ka, blockIdx.x, smem);
+#else
   prune_tile<PPL, HBM>(a, blockIdx.x, smem);
+#endif
 }
 
 // Persistent batched form: workgroups (at most 512 threads) take
@@ -494,10 +505,229 @@ __global__ __launch_bounds__(512) void prune_f32_batch_kernel(const F32Args *ins
   }
 }
 
+// Partial updates: update lists that form one chain -- entry 0 reads two
+// stored children (leaves or partials of an earlier evaluation), every later
+// entry reads the entry just before it and one stored child (the path from
+// a changed edge to the root).  The host lists the stored children in the
+// order they are read, so entry 0 reads sources 0 and 1 and entry i > 0
+// reads source i + 1: which register holds a source is known when the
+// kernel is compiled.  A wavefront issues the loads of kPathSeg sources at
+// once, runs the entries that read them from registers (one wait for the
+// whole segment instead of one memory latency per node), and moves on to
+// the next segment.  Entries and matrices are the same for every lane of a
+// wavefront and come through the scalar cache; LDS holds only the category
+// mix.  One lane per (pattern, category): wavefront (c, t) of a workgroup
+// of K x T wavefronts is category c of the t-th 64 patterns of the
+// workgroup's tile.  Arithmetic per node, root, mix and sum are
+// those of prune_tile -- products and exponent sums commute, so which side
+// carries the chain changes no bit -- and values match the other variants.
+constexpr uint32_t kPathSeg = 8;
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define B3H_CONSTANT __attribute__((address_space(4)))
+#else
+#define B3H_CONSTANT
+#endif
+typedef const B3H_CONSTANT float4 ConstF4;
+typedef const B3H_CONSTANT uint4 ConstU4;
+typedef const B3H_CONSTANT uint32_t ConstU32;
+typedef const B3H_CONSTANT F32Args ConstF32Args;
+
+__device__ __forceinline__ Mat load_mat_uniform(ConstF4 *p) {
+  Mat m;
+  m.r0 = p[0];
+  m.r1 = p[1];
+  m.r2 = p[2];
+  m.r3 = p[3];
+  return m;
+}
+
+__global__ __launch_bounds__(1024) void prune_path_kernel(F32Args arg, const uint32_t *srcs) {
+  extern __shared__ __align__(16) unsigned char smem[];
+#if defined(__HIP_DEVICE_COMPILE__)
+  // the record as the dispatch placed it (offset 0 of the kernel arguments),
+  // read through the constant address space (see prune_f32_kernel)
+  (void)arg;
+  ConstF32Args &a = *(ConstF32Args *)__builtin_amdgcn_kernarg_segment_ptr();
+#else
+  const F32Args &a = arg;
+#endif
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  const uint32_t w = uni(tid >> 6);
+  const uint32_t K = a.K, n = a.n, T = blockDim.x / (64u * K);
+  const uint32_t c = w % K, t = w / K;
+  const uint32_t p = (blockIdx.x * T + t) * 64u + lane;
+  const bool active = p < a.P;
+  const uint32_t pp = active ? p : 0u;
+  const size_t KP = (size_t)K * a.P;
+  const uint8_t *codes8 = reinterpret_cast<const uint8_t *>(a.codes32);
+  // read-only for the whole kernel: uniform reads go through the scalar cache
+  ConstU4 *ents = (ConstU4 *)a.entries;
+  ConstF4 *mats = (ConstF4 *)a.pmats;
+  ConstU32 *tab = (ConstU32 *)srcs;
+
+  // a stored child: a partial (value, exponent) or a leaf's code (in `e`)
+  auto load_src = [&](uint32_t sv, float4 &v, int &e) {
+    const uint32_t idx = sv & SRC_IDX;
+    if ((sv & SRC_TAG) == SRC_HBM) {
+      const size_t o = (size_t)idx * KP + (size_t)c * a.P + pp;
+      v = a.partials[o];
+      e = a.exps[o];
+    } else {
+      v = make_float4(0.f, 0.f, 0.f, 0.f);
+      e = codes8[(size_t)idx * a.cstride + pp];
+    }
+  };
+  auto project = [&](const Mat &M, uint32_t sv, const float4 &v, int e, float4 &out, int &oe) {
+    if ((sv & SRC_TAG) == SRC_TIP) {
+      out = tip(M, (uint32_t)e);
+      oe = 0;
+    } else {
+      out = apply(M, v);
+      oe = e;
+    }
+  };
+
+  const uint32_t s0 = tab[0];
+  float4 v0;
+  int e0;
+  load_src(s0, v0, e0);
+  float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+  int E = 0;
+  for (uint32_t i0 = 0; i0 < n; i0 += kPathSeg) {
+    uint32_t sv[kPathSeg];
+    float4 pv[kPathSeg];
+    int ev[kPathSeg];
+#pragma unroll
+    for (uint32_t j = 0; j < kPathSeg; j++) {
+      sv[j] = 0;
+      pv[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+      ev[j] = 0;
+      if (i0 + j < n) {
+        sv[j] = tab[i0 + j + 1];
+        load_src(sv[j], pv[j], ev[j]);
+      }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kPathSeg; j++) {
+      const uint32_t i = i0 + j;
+      if (i < n) {
+        const uint4 e4 = ents[i];
+        // side of the chain (entry 0: its left child, source 0)
+        const uint32_t cs = (i == 0 || (e4.y & SRC_TAG) == SRC_PREV) ? 0u : 1u;
+        const Mat MC = load_mat_uniform(mats + ((size_t)(i * 2 + cs) * K + c) * 4);
+        const Mat MS = load_mat_uniform(mats + ((size_t)(i * 2 + (cs ^ 1u)) * K + c) * 4);
+        float4 cv, sv4;
+        int ce, se;
+        if (i == 0) project(MC, s0, v0, e0, cv, ce);
+        else {
+          cv = apply(MC, x);
+          ce = E;
+        }
+        project(MS, sv[j], pv[j], ev[j], sv4, se);
+        float4 v;
+        v.x = cv.x * sv4.x;
+        v.y = cv.y * sv4.y;
+        v.z = cv.z * sv4.z;
+        v.w = cv.w * sv4.w;
+        const float mx = fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w));
+        const int ex = (mx > 0.0f) ? __builtin_amdgcn_frexp_expf(mx) : 0;
+        v.x = __builtin_amdgcn_ldexpf(v.x, -ex);
+        v.y = __builtin_amdgcn_ldexpf(v.y, -ex);
+        v.z = __builtin_amdgcn_ldexpf(v.z, -ex);
+        v.w = __builtin_amdgcn_ldexpf(v.w, -ex);
+        x = v;
+        E = ce + se + ex;
+        if (active) {
+          const size_t o = (size_t)e4.x * KP + (size_t)c * a.P + p;
+#if B3H_NT_STORE
+          const f4v xv = {v.x, v.y, v.z, v.w};
+          __builtin_nontemporal_store(xv, reinterpret_cast<f4v *>(&a.partials[o]));
+          __builtin_nontemporal_store((int16_t)E, &a.exps[o]);
+#else
+          a.partials[o] = v;
+          a.exps[o] = (int16_t)E;
+#endif
+        }
+      }
+    }
+  }
+
+  // root (cpu4.rs:277-284), then the category mix and the weighted sum
+  double pi0 = a.pi0, pi1 = a.pi1, pi2 = a.pi2, pi3 = a.pi3;
+  if (a.pik) {
+    pi0 = a.pik[4 * c];
+    pi1 = a.pik[4 * c + 1];
+    pi2 = a.pik[4 * c + 2];
+    pi3 = a.pik[4 * c + 3];
+  }
+  double site = -INFINITY;
+  if (active) {
+    double v = (double)x.x * pi0;
+    v = fma((double)x.y, pi1, v);
+    v = fma((double)x.z, pi2, v);
+    v = fma((double)x.w, pi3, v);
+    site = fma((double)E, LN2, log(v));
+    a.site[(size_t)c * a.P + p] = site;
+  }
+  // LDS: mix [T][K][64], then the sub-tile sums [T] and the ticket flag
+  double *mix = reinterpret_cast<double *>(smem);
+  double *red = mix + (size_t)T * K * 64u;
+  int *flag = reinterpret_cast<int *>(red + T);
+  double *mt = mix + (size_t)t * K * 64u;
+  mt[c * 64u + lane] = site;
+  __syncthreads();
+  if (c == 0) {
+    double wv = 0.0;
+    if (active) {
+      double mixed;
+      if (a.classes) {
+        mixed = mt[(uint32_t)a.classes[p] * 64u + lane];
+      } else {
+        double mxv = -INFINITY;
+        for (uint32_t kk = 0; kk < K; kk++) mxv = fmax(mxv, mt[kk * 64u + lane]);
+        mixed = -INFINITY;
+        if (mxv != -INFINITY) {
+          double se = 0.0;
+          for (uint32_t kk = 0; kk < K; kk++) se = se + exp(mt[kk * 64u + lane] - mxv);
+          mixed = (mxv + log(se)) + a.ln_inv_k;
+        }
+      }
+      a.mixed[p] = mixed;
+      wv = mixed * (double)a.weights[p];
+    }
+    wv = wave_sum(wv);
+    if (lane == 0) red[t] = wv;
+  }
+  __syncthreads();
+  double bsum = 0.0;
+  if (tid == 0)
+    for (uint32_t tt = 0; tt < T; tt++) bsum += red[tt];
+  last_block_total(bsum, a.blocksum, a.counter, a.total, flag, blockIdx.x, gridDim.x);
+}
+
+uint32_t path_tiles(uint32_t P, uint32_t K) {
+  const uint32_t T = K >= 16 ? 1u : 16u / K;
+  return (P + 64u * T - 1) / (64u * T);
+}
+
+hipError_t launch_prune_path(const F32Args &a, const uint32_t *srcs, hipStream_t s) {
+  if (a.K == 0 || a.K > 16 || a.n == 0) return hipErrorInvalidValue;
+  // workgroups of up to 1024 threads: T sub-tiles of 64 patterns per
+  // category, so fewer tiles take a ticket
+  const uint32_t T = a.K >= 16 ? 1u : 16u / a.K;
+  if (a.ntiles != path_tiles(a.P, a.K)) return hipErrorInvalidValue;
+  const dim3 g(a.ntiles), b(64u * a.K * T);
+  const size_t lds = (size_t)T * a.K * 64u * 8u + (size_t)T * 8u + 16u;
+  hipLaunchKernelGGL(prune_path_kernel, g, b, (uint32_t)lds, s, a, srcs);
+  return hipGetLastError();
+}
+
 hipError_t prepare_prune_f32(int max_lds_bytes) {
   const void *fns[] = {reinterpret_cast<const void *>(&prune_f32_kernel<1, false>),
                        reinterpret_cast<const void *>(&prune_f32_kernel<1, true>),
-                       reinterpret_cast<const void *>(&prune_f32_kernel<2, false>)};
+                       reinterpret_cast<const void *>(&prune_f32_kernel<2, false>),
+                       reinterpret_cast<const void *>(&prune_path_kernel)};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        max_lds_bytes);

@@ -311,3 +311,42 @@ def test_likelihood_layer_against_oracle(hip_device, K):
             lik.accept(); olik.accept(); tree.accept(); otree.accept()
         else:
             lik.reject(); olik.reject(); tree.reject(); otree.reject()
+
+
+def test_path_updates_match_full_evaluation_bitwise(hip_device):
+    """One-edge updates run in the path kernel (every in-pass child is the
+    previous entry); their site values equal a fresh handle's full
+    evaluation of the same state bit for bit, since both round each node
+    the same way."""
+    L, P, K = 60, 4000, 4
+    rng, tree_arrays, om, pm, codes, w = _setup(L, P, K, seed=23, mean_branch=0.2)
+    rates = gamma_rates(0.5, K)
+    calc = Calculator.HIP().make(codes, P, K)
+    calc.set_model(pm.q_matrix(), pm.frequencies(), True)
+    tree = DatedTree(L, *tree_arrays)
+    tree.mark_all_edges_updated()
+
+    def evaluate(c, t):
+        internals, children = t.partials_lists()
+        flat = children.reshape(-1).astype(np.int64)
+        c.propose_times(internals, children, t.heights[t.parents[flat]] - t.heights[flat], rates)
+
+    evaluate(calc, tree)
+    calc.accept()
+    tree.accept()
+    for step in range(12):
+        cand = [nd for nd in range(L, 2 * L - 1) if nd != tree.root]
+        nd = int(rng.choice(cand))
+        lo = max(tree.heights[ch] for ch in tree.children_of(nd))
+        tree.set_height(nd, float(lo + (tree.heights[tree.parent_of(nd)] - lo) * 0.4))
+        evaluate(calc, tree)
+        calc.accept()
+        fresh = Calculator.HIP().make(codes, P, K)
+        fresh.set_model(pm.q_matrix(), pm.frequencies(), True)
+        t2 = tree.copy()
+        t2.mark_all_edges_updated()
+        evaluate(fresh, t2)
+        for c in range(K):
+            np.testing.assert_array_equal(calc.likelihoods(c), fresh.likelihoods(c))
+        assert abs(calc.total() - fresh.total()) <= 1e-13 * abs(fresh.total())
+        tree.accept()
