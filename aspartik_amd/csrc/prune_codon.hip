@@ -77,7 +77,12 @@ struct Tile {
 };
 
 // LDS matrices are row-major with the 16-byte chunks of row r stored at
-// chunk ^ (r & 7), so the eight lanes of a b128 read phase hit distinct banks.
+// chunk ^ ((r >> 1) & 7).  gfx950's LDS serves a b128 read 16 lanes at a
+// time across 64 banks (256 B), and two 128-byte rows share a bank row: the
+// 16 lanes of a phase read rows 32 rb + 0..15, so rows 2q and 2q + 1 take
+// the two halves of the bank row and q selects distinct chunks.  (The
+// swizzle by r & 7 made rows r and r + 8 collide: half the LDS cycles were
+// bank conflicts.)
 
 // max / sum with the lane holding the same pattern in the other half
 __device__ __forceinline__ float pair_max(float v) {
@@ -171,7 +176,7 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
   // unit w = tid + 256 q: part w / 512, row (w / 8) % 64, 16-byte chunk w % 8
   // (rows srow and srow + 32 share the swizzle)
   const uint32_t srow = (tid >> 3) & 31u;
-  const uint32_t soff = srow * 64u + 8u * ((tid & 7u) ^ (srow & 7u));
+  const uint32_t soff = srow * 64u + 8u * ((tid & 7u) ^ ((srow >> 1) & 7u));
   auto store_mats = [&]() {
 #pragma unroll
     for (uint32_t side = 0; side < 2; side++) {
@@ -259,7 +264,7 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
     // lo parts, the 2^-20 scale, then hi parts accumulated on top; unknown
     // tips take the all-ones column (row_of(h, i) = row_of(0, i) + 4 h)
     f16v acc[2];
-    const uint32_t sw = col & 7u;
+    const uint32_t sw = (col >> 1) & 7u;
     const int lim = (int)S - 4 * (int)h;
 #pragma unroll
     for (uint32_t rb = 0; rb < 2; rb++) {
