@@ -222,6 +222,10 @@ struct b3m_chain {
   double alpha = 1.0, alpha_acc = 1.0;
   double freqs[4] = {0};
   double device_rates[6] = {-1, -1, -1, -1, -1, -1};  // model currently on the GPU
+  // category rates of cat_alpha (the discretised gamma costs tens of
+  // microseconds on the host, and alpha changes only on its own moves)
+  double cat_alpha = -1.0;
+  double cat_rates[64] = {0};
   // accepted values
   double loglik = 0.0, logprior = 0.0, posterior = 0.0;
   double surrogate_cur = 0.0;
@@ -280,11 +284,13 @@ b3h_status evaluate(b3m_chain *c, double *out) {
   const size_t n = c->internals.size();
   c->times.resize(2 * n);
   for (size_t m = 0; m < 2 * n; m++) c->times[m] = c->tree.edge_length(c->kids[m]);
-  double cat[64];
-  b3h_status st = b3h_gamma_rates(c->alpha, c->K, cat);
-  if (st) return st;
+  b3h_status st = B3H_OK;
+  if (c->alpha != c->cat_alpha) {
+    if ((st = b3h_gamma_rates(c->alpha, c->K, c->cat_rates))) return st;
+    c->cat_alpha = c->alpha;
+  }
   st = b3h_propose_times(c->calc, (uint32_t)n, c->internals.data(), c->kids.data(),
-                         c->times.data(), cat);
+                         c->times.data(), c->cat_rates);
   if (st) return st;
   return b3h_total(c->calc, out);
 }

@@ -199,33 +199,73 @@ __global__ void k_adam(float *p, const float *g, float *m, float *v, uint16_t *p
 // One sample, one workgroup: the whole forward pass with the input passed by
 // value (no copy to the device) and the result written to pinned host
 // memory (no copy back): one launch per surrogate call in the MCMC loop.
-// The hidden units use the same FP32 summation order as k_dense_fwd.
+// The hidden units use the same FP32 summation order as k_dense_fwd.  The
+// weights are read from transposed copies (one padded row of BF16 per
+// hidden unit, k_transpose), so a lane loads eight weights per 16-byte load
+// and all of a row's loads are independent: in the [k][n] layout of the
+// training kernels a lane read two bytes per load, and the pass was bound
+// by one CU's memory latency (about 40 us per call).
 constexpr uint32_t kMaxFusedD = 512, kMaxFusedH = 1024;
 struct FusedInput {
   float x[kMaxFusedD];
 };
-__global__ __launch_bounds__(256) void k_predict1(FusedInput in, uint32_t D, uint32_t H,
-                                                  const float *xmean, const float *xstd,
-                                                  const uint16_t *W1, const float *b1,
-                                                  const uint16_t *W2, const float *b2,
-                                                  const uint16_t *w3, const float *b3, float ymean,
-                                                  float ystd, float *out) {
+// row of weights (BF16, zero-padded from `used` to a multiple of 8) dotted
+// with v in index order, starting from z: whole groups of eight weights,
+// sixteen 16-byte loads in flight at a time, then the last partial group
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+__device__ __forceinline__ float dot8(const uint4 w, const float *v, float z) {
+  z = fmaf(v[0], bf16_lo(w.x), z);
+  z = fmaf(v[1], bf16_hi(w.x), z);
+  z = fmaf(v[2], bf16_lo(w.y), z);
+  z = fmaf(v[3], bf16_hi(w.y), z);
+  z = fmaf(v[4], bf16_lo(w.z), z);
+  z = fmaf(v[5], bf16_hi(w.z), z);
+  z = fmaf(v[6], bf16_lo(w.w), z);
+  return fmaf(v[7], bf16_hi(w.w), z);
+}
+__device__ __forceinline__ float dot_row(const uint16_t *row, const float *v, uint32_t used,
+                                         float z) {
+  const uint4 *r = reinterpret_cast<const uint4 *>(row);
+  const uint32_t full = used / 8u;
+  uint32_t g = 0;
+  constexpr uint32_t kBatch = 16;  // 16-byte loads in flight per lane
+  for (; g + kBatch <= full; g += kBatch) {
+    uint4 w[kBatch];
+#pragma unroll
+    for (uint32_t q = 0; q < kBatch; q++) w[q] = r[g + q];
+#pragma unroll
+    for (uint32_t q = 0; q < kBatch; q++) z = dot8(w[q], v + 8u * (g + q), z);
+  }
+  for (; g < full; g++) z = dot8(r[g], v + 8u * g, z);
+  if (used % 8u) {
+    const uint4 w = r[full];
+    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+    for (uint32_t j = 0; j < used % 8u; j++)
+      z = fmaf(v[8u * full + j], (j & 1u) ? bf16_hi(ws[j / 2]) : bf16_lo(ws[j / 2]), z);
+  }
+  return z;
+}
+
+__global__ __launch_bounds__(256) void k_predict1(FusedInput in, uint32_t D, uint32_t Dp,
+                                                  uint32_t H, uint32_t Hp, const float *xmean,
+                                                  const float *xstd, const uint16_t *W1t,
+                                                  const float *b1, const uint16_t *W2t,
+                                                  const float *b2, const uint16_t *w3,
+                                                  const float *b3, float ymean, float ystd,
+                                                  float *out) {
   __shared__ float xs[kMaxFusedD];
   __shared__ float h1[kMaxFusedH];
   __shared__ float part[4];
   const uint32_t tid = threadIdx.x;
   for (uint32_t k = tid; k < D; k += 256u) xs[k] = (in.x[k] - xmean[k]) / xstd[k];
   __syncthreads();
-  for (uint32_t n = tid; n < H; n += 256u) {
-    float z = b1[n];
-    for (uint32_t k = 0; k < D; k++) z = fmaf(xs[k], from_bf16(W1[(size_t)k * H + n]), z);
-    h1[n] = gelu(z);
-  }
+  for (uint32_t n = tid; n < H; n += 256u)
+    h1[n] = gelu(dot_row(W1t + (size_t)n * Dp, xs, D, b1[n]));
   __syncthreads();
   float s = 0.0f;
   for (uint32_t n = tid; n < H; n += 256u) {
-    float z = b2[n];
-    for (uint32_t k = 0; k < H; k++) z = fmaf(h1[k], from_bf16(W2[(size_t)k * H + n]), z);
+    const float z = dot_row(W2t + (size_t)n * Hp, h1, H, b2[n]);
     s = fmaf(gelu(z), from_bf16(w3[n]), s);
   }
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
@@ -239,6 +279,14 @@ __global__ void k_cast(const float *p, uint16_t *pb, size_t n) {
   if (i < n) pb[i] = to_bf16(p[i]);
 }
 
+// W [K][N] (BF16) -> Wt [N][Kp], zero-padded from K to Kp
+__global__ void k_transpose(const uint16_t *W, uint16_t *Wt, uint32_t K, uint32_t N, uint32_t Kp) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)N * Kp) return;
+  const uint32_t n = (uint32_t)(i / Kp), k = (uint32_t)(i % Kp);
+  Wt[i] = k < K ? W[(size_t)k * N + n] : (uint16_t)0;
+}
+
 }  // namespace
 
 struct b3m_mlp {
@@ -248,6 +296,11 @@ struct b3m_mlp {
   hipStream_t stream = nullptr;
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   uint16_t *pb = nullptr;  // BF16 copy (all parameters; the biases' entries unused)
+  // transposed BF16 W1 [H][Dp] and W2 [H][Hp] for the single-sample pass,
+  // rebuilt from pb before the first prediction after the parameters change
+  uint16_t *w1t = nullptr, *w2t = nullptr;
+  uint32_t Dp = 0, Hp = 0;
+  bool t_stale = true;
   float *xmean = nullptr, *xstd = nullptr;
   float ymean = 0.0f, ystd = 1.0f;
   uint32_t bcap = 0;
@@ -342,10 +395,19 @@ hipError_t backward(b3m_mlp *m, uint32_t B) {
 namespace b3h {
 b3h_status mlp_predict(b3m_mlp *m, const float *x, uint32_t B, float *y) {
   if (B == 1 && m->D <= kMaxFusedD && m->H <= kMaxFusedH && m->h_out) {
+    if (m->t_stale) {
+      const size_t n1 = (size_t)m->H * m->Dp, n2 = (size_t)m->H * m->Hp;
+      hipLaunchKernelGGL(k_transpose, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, m->stream,
+                         m->pb + m->oW1, m->w1t, m->D, m->H, m->Dp);
+      hipLaunchKernelGGL(k_transpose, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, m->stream,
+                         m->pb + m->oW2, m->w2t, m->H, m->H, m->Hp);
+      MCHK(hipGetLastError());
+      m->t_stale = false;
+    }
     FusedInput in;
     std::memcpy(in.x, x, (size_t)m->D * 4);
-    hipLaunchKernelGGL(k_predict1, dim3(1), dim3(256), 0, m->stream, in, m->D, m->H, m->xmean,
-                       m->xstd, m->pb + m->oW1, m->p + m->ob1, m->pb + m->oW2, m->p + m->ob2,
+    hipLaunchKernelGGL(k_predict1, dim3(1), dim3(256), 0, m->stream, in, m->D, m->Dp, m->H,
+                       m->Hp, m->xmean, m->xstd, m->w1t, m->p + m->ob1, m->w2t, m->p + m->ob2,
                        m->pb + m->ow3, m->p + m->ob3, m->ymean, m->ystd, m->h_out);
     MCHK(hipGetLastError());
     MCHK(hipStreamSynchronize(m->stream));
@@ -384,6 +446,8 @@ b3h_status b3m_mlp_create(int device, uint32_t inputs, uint32_t hidden, uint64_t
   m->device = device;
   m->D = inputs;
   m->H = hidden;
+  m->Dp = (inputs + 7u) & ~7u;
+  m->Hp = (hidden + 7u) & ~7u;
   m->seed = seed;
   m->oW1 = 0;
   m->ob1 = (size_t)inputs * hidden;
@@ -404,6 +468,8 @@ b3h_status b3m_mlp_create(int device, uint32_t inputs, uint32_t hidden, uint64_t
       (st = malloc_dev(m, (void **)&m->m, m->np * 4)) ||
       (st = malloc_dev(m, (void **)&m->v, m->np * 4)) ||
       (st = malloc_dev(m, (void **)&m->pb, m->np * 2)) ||
+      (st = malloc_dev(m, (void **)&m->w1t, (size_t)hidden * ((inputs + 7u) & ~7u) * 2)) ||
+      (st = malloc_dev(m, (void **)&m->w2t, (size_t)hidden * ((hidden + 7u) & ~7u) * 2)) ||
       (st = malloc_dev(m, (void **)&m->xmean, (size_t)inputs * 4)) ||
       (st = malloc_dev(m, (void **)&m->xstd, (size_t)inputs * 4)) ||
       (st = malloc_dev(m, (void **)&m->loss, 16)))
@@ -481,6 +547,7 @@ b3h_status b3m_mlp_set_params(b3m_mlp *m, const float *params) {
   MCHK(hipMemcpy(m->p, params, m->np * 4, hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_cast, dim3((unsigned)((m->np + 255) / 256)), dim3(256), 0, m->stream,
                      m->p, m->pb, m->np);
+  m->t_stale = true;
   MCHK(hipStreamSynchronize(m->stream));
   return B3H_OK;
 }
@@ -559,6 +626,7 @@ b3h_status b3m_mlp_train(b3m_mlp *m, const float *x, const float *y, uint32_t n,
       const float c2 = 1.0f - std::pow(b2, (float)m->adam_t);
       hipLaunchKernelGGL(k_adam, dim3((unsigned)((m->np + 255) / 256)), dim3(256), 0, m->stream,
                          m->p, m->g, m->m, m->v, m->pb, m->np, lr, b1, b2, 1e-8f, c1, c2);
+      m->t_stale = true;
       // synchronise before the next index upload reuses the host order buffer
       if (hipStreamSynchronize(m->stream) != hipSuccess)
         return done(b3h::host_fail(B3H_ERR_HIP, "surrogate training failed"));

@@ -81,6 +81,20 @@ def test_single_sample_path_matches_batched(hip_device):
     batched = s.forward(x)
     single = np.array([s.forward(x[i:i + 1])[0] for i in range(9)])
     np.testing.assert_allclose(single, batched, rtol=1e-6, atol=1e-4)
+    # after training steps the one-sample path reads the updated weights (its
+    # transposed copy is rebuilt), and a width that is not a multiple of 8
+    # exercises the padded tail of each row
+    s.train(x, rng.normal(size=9).astype(np.float32), epochs=3, batch=9)
+    batched = s.forward(x)
+    single = np.array([s.forward(x[i:i + 1])[0] for i in range(9)])
+    np.testing.assert_allclose(single, batched, rtol=1e-6, atol=1e-4)
+    s.close()
+    t = Surrogate(37, 45, seed=3)
+    t.set_scaling(np.zeros(37), np.ones(37), 0.0, 1.0)
+    z = rng.normal(size=(5, 37)).astype(np.float32)
+    np.testing.assert_allclose([t.forward(z[i:i + 1])[0] for i in range(5)], t.forward(z),
+                               rtol=1e-6, atol=1e-5)
+    t.close()
 
 
 def test_surrogate_gradient_is_a_derivative(hip_device):
