@@ -27,12 +27,6 @@ b3h_status fail(b3h_status s, const std::string &msg) {
 
 }  // namespace
 
-// Route eligible FP32 partial-update lists to the path kernel (1) or keep
-// them on the fused pruning kernel (0).
-#ifndef B3H_PATH_KERNEL
-#define B3H_PATH_KERNEL 1
-#endif
-
 namespace b3h {
 // shared with the other host translation units (batch.hip)
 b3h_status host_fail(b3h_status s, const std::string &msg) { return fail(s, msg); }
@@ -252,7 +246,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
   bool path = false;
   std::vector<uint32_t> &psrc = c->path_srcs;
   psrc.clear();
-  if (B3H_PATH_KERNEL && f32 && n <= 64) {
+  if (f32 && n <= 64) {
     if (!c->sched.build(n, internals, children, c->L, c->editbuf_next.data(), true, 0, &err))
       return fail(B3H_ERR_ARG, err);
     path = true;
