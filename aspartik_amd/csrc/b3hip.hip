@@ -80,6 +80,7 @@ struct b3h_calc {
   // staging (pinned host -> device), double buffered
   size_t stage_bytes = 0;
   unsigned char *h_stage[kStages] = {nullptr, nullptr};
+  double *h_total = nullptr;  // pinned target of the total's read-back
   unsigned char *d_stage[kStages] = {nullptr, nullptr};
   hipEvent_t stage_ev[kStages] = {nullptr, nullptr};
   int stage_next = 0;
@@ -135,6 +136,7 @@ void release(b3h_calc *c) {
                   c->d_counter, c->d_total, c->d_pmats, c->d_model, c->d_ucodon};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->h_total) (void)hipHostFree(c->h_total);
   for (int s = 0; s < kStages; s++) {
     if (c->h_stage[s]) (void)hipHostFree(c->h_stage[s]);
     if (c->d_stage[s]) (void)hipFree(c->d_stage[s]);
@@ -818,6 +820,10 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
                    ((size_t)K * 2 * c->I * host_mat + 255) / 256 * 256 +
                    ((size_t)K * 2 * c->I * 8 + 255) / 256 * 256 + ((size_t)K * 8 + 255) / 256 * 256 +
                    64 * 8 + 512;
+  if (hipHostMalloc((void **)&c->h_total, 64, hipHostMallocDefault) != hipSuccess) {
+    fail(B3H_ERR_OOM, "hipHostMalloc failed");
+    return bail(B3H_ERR_OOM);
+  }
   for (int s = 0; s < kStages; s++) {
     if (hipHostMalloc((void **)&c->h_stage[s], c->stage_bytes, hipHostMallocDefault) != hipSuccess) {
       fail(B3H_ERR_OOM, "hipHostMalloc staging failed");
@@ -1115,8 +1121,11 @@ b3h_status b3h_total(b3h_calc *c, double *out) {
   uint32_t slot;
   if ((st = result_slot(c, &slot))) return st;
   if ((st = set_device(c))) return st;
-  HIPCHK(hipMemcpyAsync(out, c->d_total + slot, 8, hipMemcpyDeviceToHost, c->stream));
+  // into pinned memory: a pageable destination goes through HIP's own
+  // staging copy
+  HIPCHK(hipMemcpyAsync(c->h_total, c->d_total + slot, 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  *out = *c->h_total;
   return B3H_OK;
 }
 
