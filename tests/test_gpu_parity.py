@@ -350,3 +350,49 @@ def test_path_updates_match_full_evaluation_bitwise(hip_device):
             np.testing.assert_array_equal(calc.likelihoods(c), fresh.likelihoods(c))
         assert abs(calc.total() - fresh.total()) <= 1e-13 * abs(fresh.total())
         tree.accept()
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_total_over_many_tiles_equals_site_sum(hip_device, precision):
+    """The device total comes from a last-tile reduction whose ticket orders
+    each tile's partial sum with agent-scope atomics and no cache
+    write-back.  At 200,000 patterns a full evaluation has about 400 tiles,
+    a one-edge update in the path kernel 782, and the FP64 mix kernel 782:
+    over repeated evaluations, accepted and rejected, every total equals the
+    FP64 weighted sum of the per-pattern values the same evaluation wrote
+    (to summation-order rounding), so no tile's partial was missed or
+    stale."""
+    L, P, K = 24, 200_000, 4
+    rng, tree_arrays, om, pm, codes, w = _setup(L, P, K, seed=31, unknown=0.02)
+    rates = gamma_rates(0.5, K)
+    calc = (Calculator.HIP() if precision == "f32" else Calculator.HIPPlainF64()).make(codes, P, K)
+    calc.set_weights(w)
+    calc.set_model(pm.q_matrix(), pm.frequencies(), True)
+    tree = DatedTree(L, *tree_arrays)
+
+    def evaluate():
+        internals, children = tree.partials_lists()
+        flat = children.reshape(-1).astype(np.int64)
+        calc.propose_times(internals, children,
+                           tree.heights[tree.parents[flat]] - tree.heights[flat], rates)
+        total = calc.total()
+        expect = float(np.dot(calc.site_mixed(), w.astype(np.float64)))
+        assert abs(total - expect) <= 1e-12 * abs(expect), (total, expect)
+        return total
+
+    tree.mark_all_edges_updated()
+    evaluate()
+    calc.accept()
+    tree.accept()
+    for step in range(8):
+        cand = [nd for nd in range(L, 2 * L - 1) if nd != tree.root]
+        nd = int(rng.choice(cand))
+        lo = max(tree.heights[ch] for ch in tree.children_of(nd))
+        tree.set_height(nd, float(lo + (tree.heights[tree.parent_of(nd)] - lo) * 0.6))
+        evaluate()
+        if step % 3 == 2:
+            calc.reject()
+            tree.reject()
+        else:
+            calc.accept()
+            tree.accept()
