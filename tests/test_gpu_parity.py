@@ -313,14 +313,16 @@ def test_likelihood_layer_against_oracle(hip_device, K):
             lik.reject(); olik.reject(); tree.reject(); otree.reject()
 
 
-def test_path_updates_match_full_evaluation_bitwise(hip_device):
+@pytest.mark.parametrize("K", [1, 3, 4, 16])
+def test_path_updates_match_full_evaluation_bitwise(hip_device, K):
     """One-edge updates run in the path kernel (every in-pass child is the
     previous entry); their site values equal a fresh handle's full
     evaluation of the same state bit for bit, since both round each node
-    the same way."""
-    L, P, K = 60, 4000, 4
+    the same way.  K sets the path kernel's workgroup (16 / K sub-tiles of
+    64 patterns: 1024, 960, 1024 and 1024 threads)."""
+    L, P = 60, 4000
     rng, tree_arrays, om, pm, codes, w = _setup(L, P, K, seed=23, mean_branch=0.2)
-    rates = gamma_rates(0.5, K)
+    rates = gamma_rates(0.5, K) if K > 1 else np.ones(1)
     calc = Calculator.HIP().make(codes, P, K)
     calc.set_model(pm.q_matrix(), pm.frequencies(), True)
     tree = DatedTree(L, *tree_arrays)
