@@ -8,11 +8,11 @@ from .parameters import ClassVector, IntVector, Real, RealVector
 from .mcmc import Chain, Surrogate, ess
 from .substitution import GTR, GY94, HKY, JC, K80, SENSE_CODONS
 from .tree import TOMBSTONE, DatedTree
-from .tuning import autotune
+from .tuning import autotune, autotune_batch
 
 __all__ = [
     "Batch", "Calculator", "HipCalculator", "gamma_rates", "DNALikelihood", "GammaLikelihood",
     "HeteroLikelihood", "Clock", "StrictClock", "RelaxedClock", "ClassVector", "Real", "RealVector", "IntVector",
     "deduplicate", "GTR", "GY94", "HKY", "JC", "K80", "SENSE_CODONS", "TOMBSTONE",
-    "DatedTree", "Chain", "Surrogate", "ess", "autotune",
+    "DatedTree", "Chain", "Surrogate", "ess", "autotune", "autotune_batch",
 ]

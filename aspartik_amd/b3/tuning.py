@@ -118,3 +118,68 @@ def autotune(codes, weights, tree: DatedTree, substitution, category_rates,
                             fk / max(1, fn) * 1e3, pk / max(1, pn) * 1e3))
     trials.sort(key=lambda tr: tr.score_ms)
     return trials[0].config, trials
+
+
+# ---- batched evaluation ------------------------------------------------------
+
+@dataclass
+class BatchTrial:
+    tile_patterns: int   # patterns per lane row of a workgroup's tile (64 x sub-tiles)
+    lane_patterns: int   # patterns per lane
+    ms: float            # per batch evaluation: captured-graph replay, synchronous
+    kernel_ms: float     # the batched pruning kernel (HIP events)
+
+
+def batch_candidates(num_categories: int):
+    """Tile shapes of the batched kernel (workgroups of at most 512 threads):
+    every power-of-two number of 64-pattern sub-tiles, one or two patterns
+    per lane."""
+    t = max(1, 512 // (64 * num_categories))
+    out = []
+    while t >= 1:
+        out += [(64 * t, 1), (64 * t, 2)]
+        t //= 2
+    return out
+
+
+def autotune_batch(instances, models, updates, num_categories: int, evals: int = 8,
+                   candidates=None, single_slot: bool = True, device: int = 0):
+    """Measured tile choice for one batch: each candidate (tile_patterns,
+    lane_patterns) evaluates the caller's own instances -- one eager
+    evaluation, then ``evals`` captured-graph replays -- and the fastest
+    wins.  instances: [(codes, weights)]; models: [(q, frequencies,
+    special_cases)] per instance; updates: one update per instance, as for
+    ``Batch.propose_times``.  Returns ((tile_patterns, lane_patterns),
+    trials sorted fastest first).  Every shape computes the same per-site
+    values; only the schedule differs."""
+    from .batch import Batch
+    trials = []
+    for tp, lp in (candidates or batch_candidates(num_categories)):
+        b = Batch(instances, num_categories, device, single_slot, tile_patterns=tp,
+                  lane_patterns=lp)
+        try:
+            for i, (q, f, special) in enumerate(models):
+                b.set_model(i, q, f, special)
+            b.propose_times(updates)
+            b.totals()
+            b.accept()
+            b.propose_times(updates, graph=True)
+            b.totals()
+            b.accept()
+            t0 = time.perf_counter()
+            for _ in range(evals):
+                b.propose_times(updates, graph=True)
+                b.totals()
+                b.accept()
+            ms = (time.perf_counter() - t0) / evals * 1e3
+            b.profile_enable(True)
+            b.propose_times(updates)
+            b.totals()
+            b.accept()
+            kms, kn = b.profile_read()
+            b.profile_enable(False)
+        finally:
+            b.close()
+        trials.append(BatchTrial(tp, lp, ms, kms / max(1, kn)))
+    trials.sort(key=lambda tr: tr.ms)
+    return (trials[0].tile_patterns, trials[0].lane_patterns), trials

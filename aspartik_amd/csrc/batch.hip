@@ -174,9 +174,10 @@ b3h_status b3h_batch_create(const b3h_config *cfg, uint32_t count, const b3h_ins
   b->T = std::max<uint32_t>(1u, 8u / K);  // 512-thread workgroups
   while (64u * K * b->T > 512u && b->T > 1) b->T--;
   if (cfg->tile_patterns) b->T = std::max<uint32_t>(1u, std::min(cfg->tile_patterns / 64u, 512u / (64u * K)));
-  // one pattern per lane measured fastest for batches (32.0 ms vs 34.9 ms
-  // for two, 256 HKY+Gamma4 instances)
-  b->PPL = cfg->lane_patterns == 2 ? 2u : 1u;
+  // two patterns per lane unless asked for one: 256 HKY+Gamma4 instances
+  // measured 29.5 ms per batch against 33.9 ms with one (the measured choice
+  // per workload is tuning.autotune_batch)
+  b->PPL = cfg->lane_patterns == 1 ? 1u : 2u;
   if (hipSetDevice(b->device) != hipSuccess ||
       hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     host_fail(B3H_ERR_HIP, "stream creation failed");

@@ -83,20 +83,28 @@ def bench_batch(args):
     every instance fully evaluated in one persistent launch per step,
     replayed from a captured HIP graph."""
     from aspartik_amd import workloads
-    from aspartik_amd.b3 import Batch, DatedTree
+    from aspartik_amd.b3 import Batch, DatedTree, autotune_batch
 
     log("[bench] building batched workload")
     wl = workloads.hky_gamma4_batch(count=args.instances)
     insts = wl["instances"]
     K, rates = wl["K"], wl["rates"]
-    b = Batch([(x["codes"], None) for x in insts], num_categories=K, single_slot=True)
     ups, B = [], 0
-    for i, x in enumerate(insts):
-        b.set_model(i, x["model"].q_matrix(), x["model"].frequencies(), True)
+    for x in insts:
         t = DatedTree(x["L"], x["parents"], x["children"], x["heights"])
         internals, children = workloads.full_lists(t)
         ups.append((internals, children, workloads.edge_times(t, children), rates))
         B += workloads.algorithmic_bytes(internals, children, x["L"], t.root, K, x["P"])
+    data = [(x["codes"], None) for x in insts]
+    models = [(x["model"].q_matrix(), x["model"].frequencies(), True) for x in insts]
+    log("[bench] batched: tuning tiles")
+    (tile, lanes), trials = autotune_batch(data, models, ups, K, evals=6)
+    for tr in trials:
+        log(f"[bench]   tile {tr.tile_patterns} x {tr.lane_patterns}/lane: {tr.ms:.2f} ms per "
+            f"batch, kernel {tr.kernel_ms:.2f} ms")
+    b = Batch(data, num_categories=K, single_slot=True, tile_patterns=tile, lane_patterns=lanes)
+    for i, (q, f, special) in enumerate(models):
+        b.set_model(i, q, f, special)
     log("[bench] batched: warmup")
     for _ in range(args.warmup):
         b.propose_times(ups, graph=True)
@@ -162,13 +170,16 @@ def bench_batch(args):
         "data": "synthetic (seeded Yule trees, sequences simulated along them)",
         "config": {"workload": f"hky_gamma4_batch{args.instances}", "instances": args.instances,
                    "taxa": "U{50..500}", "patterns": "U{2000..50000}", "categories": K,
-                   "evaluation": "full, all instances per step, graph replay, single slot"},
+                   "evaluation": "full, all instances per step, graph replay, single slot",
+                   "tile_patterns": tile, "lane_patterns": lanes},
         "roofline": {"bound": "hbm", "achieved": B / (kernel_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": B / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu, "kernel_ms": kernel_ms, "algorithmic_bytes_per_eval": B,
         "batch_evals_per_s": args.steps / dt, "error_vs_fp64_oracle": err,
         "device_bytes": b.device_bytes(),
+        "tile_trials": [{"tile_patterns": tr.tile_patterns, "lane_patterns": tr.lane_patterns,
+                         "ms_per_batch": tr.ms, "kernel_ms": tr.kernel_ms} for tr in trials],
     }), flush=True)
 
 

@@ -158,3 +158,33 @@ def test_batch_graph_replay_matches_eager(hip_device):
     b.accept()
     b.propose_times(scaled, graph=True)
     np.testing.assert_array_equal(b.totals(), eager2)
+
+
+def test_batch_autotune_picks_a_candidate_with_identical_sites(hip_device):
+    """The batch tile autotuner times every candidate on the caller's own
+    instances and returns the fastest; whichever it picks, per-site values
+    equal the default shape's bit for bit (only the schedule differs)."""
+    from aspartik_amd.b3 import autotune_batch
+    from aspartik_amd.b3.tuning import batch_candidates
+    K = 4
+    rates = gamma_rates(0.5, K)
+    insts = _instances(7, [(30, 1500), (80, 700), (12, 3000)])
+    data = [(x["codes"], x["w"]) for x in insts]
+    models = [(x["model"].q_matrix(), x["model"].frequencies(), True) for x in insts]
+    ups = [_full_update(x, rates) for x in insts]
+    (tile, lanes), trials = autotune_batch(data, models, ups, K, evals=2)
+    assert (tile, lanes) in batch_candidates(K)
+    assert len(trials) == len(batch_candidates(K)) and trials[0].ms <= trials[-1].ms
+    results = []
+    for tp, lp in [(0, 0), (tile, lanes)]:
+        b = Batch(data, num_categories=K, single_slot=True, tile_patterns=tp, lane_patterns=lp)
+        for i, (q, f, sc) in enumerate(models):
+            b.set_model(i, q, f, sc)
+        b.propose_times(ups)
+        results.append([[b.likelihoods(i, k) for k in range(K)] for i in range(len(insts))])
+        totals = b.totals()
+        b.close()
+        assert np.all(np.isfinite(totals))
+    for a, c in zip(*results):
+        for x, y in zip(a, c):
+            np.testing.assert_array_equal(x, y)
