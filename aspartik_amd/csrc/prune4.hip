@@ -514,14 +514,14 @@ __global__ __launch_bounds__(512) void prune_f32_batch_kernel(const F32Args *ins
 // kernel is compiled.  A wavefront issues the loads of kPathSeg sources at
 // once, runs the entries that read them from registers (one wait for the
 // whole segment instead of one memory latency per node), and moves on to
-// the next segment.  (Eight waves per SIMD hide the rest: at 200 taxa and
+// the next segment; eight waves per SIMD hide the rest (at 200 taxa and
 // 200,000 patterns, segments of 2-4 sources measured 58-59 us, 8 sources
-// 70 us, 16 sources 66 us.)  Entries and matrices are the same for every lane of a
-// wavefront and come through the scalar cache; LDS holds only the category
-// mix.  One lane per (pattern, category): wavefront (c, t) of a workgroup
-// of K x T wavefronts is category c of the t-th 64 patterns of the
-// workgroup's tile.  Arithmetic per node, root, mix and sum are
-// those of prune_tile -- products and exponent sums commute, so which side
+// 70 us, 16 sources 66 us).  Entries and matrices are the same for every
+// lane of a wavefront and come through the scalar cache; LDS holds only the
+// category mix.  One lane per (pattern, category): wavefront (c, t) of a
+// workgroup of K x T wavefronts is category c of the t-th 64 patterns of
+// the workgroup's tile.  Arithmetic per node, root, mix and sum are those
+// of prune_tile -- products and exponent sums commute, so which side
 // carries the chain changes no bit -- and values match the other variants.
 constexpr uint32_t kPathSeg = 4;
 
