@@ -770,12 +770,26 @@ def main():
             if time.perf_counter() - t0 > args.cpu_seconds or nev >= 200:
                 break
         dtc = time.perf_counter() - t0
+        # one-edge partial evaluations over the GPU leg's edges (the edge and
+        # its ancestors, tree.rs:357-375)
+        npc = 0
+        t0 = time.perf_counter()
+        while True:
+            otree.mark_edge_updated(int(edges[npc % len(edges)]))
+            olik.likelihood()
+            olik.accept()
+            otree.accept()
+            npc += 1
+            if time.perf_counter() - t0 > args.cpu_seconds / 3 or npc >= 300:
+                break
+        dtp = time.perf_counter() - t0
         cpu = {"value": nev / dtc, "unit": "logL evals/s", "cores": tpc * K,
-               "kind": "port",
+               "kind": "port", "partial_evals_per_s": npc / dtp,
                "sample": f"{nev} full GTR+Gamma4 evaluations of the same {L}-taxon "
-                         f"{P}-pattern workload ({dtc:.1f} s; {K} categories x {tpc} "
+                         f"{P}-pattern workload ({dtc:.1f} s) and {npc} one-edge partial "
+                         f"evaluations ({dtp:.1f} s); {K} categories x {tpc} "
                          f"threads as cpu4.rs/gamma.rs; host has {ncores} cores; "
-                         f"construction {t_first:.1f} s)"}
+                         f"construction {t_first:.1f} s"}
         rel_total = abs(total32 - ref_total) / abs(ref_total)
         per_site = np.abs(site32 - ref_sites) / np.maximum(1.0, np.abs(ref_sites))
         err = dict(total_rel=rel_total, max_site=float(per_site.max()), ref_total=ref_total,
