@@ -279,7 +279,9 @@ b3h_status evaluate(b3m_chain *c, double *out) {
     if (st) return st;
     for (int i = 0; i < 6; i++) c->device_rates[i] = c->rates[i];
   }
-  if (!c->tree.changed()) c->tree.mark_all();  // a parameter move: every P(t) changes
+  // a parameter move changes every P(t); full_evaluations recomputes every
+  // node for any move
+  if (c->cfg.full_evaluations || !c->tree.changed()) c->tree.mark_all();
   c->tree.partials_lists(c->internals, c->kids);
   const size_t n = c->internals.size();
   c->times.resize(2 * n);

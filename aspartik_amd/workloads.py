@@ -162,6 +162,26 @@ def gy94_codon(num_leaves=64, num_patterns=20_000, omega=0.3, kappa=2.0, alpha=0
                 seed=seed)
 
 
+def hky_cpu_mcmc(num_leaves=16, num_sites=1_000, kappa=2.0, seed=4):
+    """The reference's CPU-runnable MCMC case: a seeded Yule tree (birth rate
+    1, branch lengths rescaled to mean 0.1), HKY with kappa 2 and equal base
+    frequencies, ``num_sites`` columns simulated on the tree and compressed to
+    unique patterns with their counts as weights (first-occurrence order)."""
+    from .data import deduplicate
+    rng = np.random.default_rng(seed)
+    parents, children, heights = sim.yule_tree(num_leaves, rng, 1.0, 0.1)
+    freqs = np.full(4, 0.25)
+    model = HKY(freqs, kappa)
+    states, _ = sim.simulate_states(parents, children, heights,
+                                    sim.expm_transition(model.q_matrix()), freqs, np.ones(1),
+                                    num_sites, rng)
+    codes, weights, _ = deduplicate(sim.dna_codes(states))
+    return dict(name=f"hky_{num_leaves}taxa_{num_sites}sites_mcmc", parents=parents,
+                children=children, heights=heights, model=model, codes=codes,
+                weights=weights, K=1, kappa=kappa, freqs=freqs,
+                gtr_rates=np.array([1.0, kappa, 1.0, 1.0, kappa, 1.0]), seed=seed)
+
+
 def gtr_gamma4_mcmc(num_leaves=200, num_sites=200_000, alpha=0.5, seed=4):
     """MCMC workload: GTR rates ~ Dirichlet(1), frequencies ~ Dirichlet(10),
     Gamma4 alpha 0.5, ``num_sites`` sites simulated on a seeded Yule tree

@@ -78,6 +78,75 @@ def time_evals(calc, lists, times, rates, steps, warmup, sync_each=True):
     return dt, total, kms / max(1, launches)
 
 
+def bench_c1(args):
+    """The reference's CPU-runnable case: 16 taxa (seeded Yule tree), HKY with
+    kappa 2 and equal frequencies, 1,000 simulated sites compressed to unique
+    patterns, 10,000 plain MCMC steps with every node recomputed at each
+    likelihood call -- timed end to end with the native chain driver
+    (NodeSlide, TreeScale, NarrowExchange at b3's weights; the substitution
+    model fixed) and its GPU likelihood.  cpu_baseline: the oracle's FP64
+    likelihood (cpu4.rs threading: one thread below 1,000 patterns) over the
+    same number of full evaluations, the likelihood work of such a run.  At
+    this size a step is launch- and synchronisation-bound on the GPU."""
+    from aspartik_amd import workloads
+    from aspartik_amd.b3 import Chain
+
+    wl = workloads.hky_cpu_mcmc(seed=4)
+    L, P = wl["codes"].shape
+    steps = args.c1_steps
+    log(f"[bench] C1: {L} taxa, {P} patterns (1,000 sites), {steps} steps")
+    c = Chain(wl["codes"], wl["parents"], wl["children"], wl["heights"], wl["gtr_rates"],
+              wl["freqs"], 1.0, weights=wl["weights"], categories=1, w_rates=0.0, w_alpha=0.0,
+              full_evaluations=1, seed=4)
+    c.run(200)
+    t0 = time.perf_counter()
+    _, st, _ = c.run(steps)
+    dt = time.perf_counter() - t0
+    loglik = c.state()[0]
+    par, ch, h, _, _ = c.params()
+    c.close()
+    log(f"[bench] C1 chain: {dt:.2f} s for {steps} steps; accepted {st['accepted']}")
+    cpu, err = None, None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ot = O.Tree(L, par, ch.astype(np.uint32), h)
+        om = O.Model.hky(wl["kappa"], wl["freqs"])
+        lik = O.GammaLikelihood(ot, om, wl["codes"], P, wl["weights"], num_categories=1,
+                                threads_per_category=1)
+        ref = lik.likelihood()
+        lik.accept()
+        ot.accept()
+        err = {"final_state_rel": abs(loglik - ref) / abs(ref), "ref": ref, "gpu": loglik}
+        nev = 0
+        t0 = time.perf_counter()
+        while nev < steps and time.perf_counter() - t0 < args.cpu_seconds:
+            ot.mark_all_edges_updated()
+            lik.likelihood()
+            lik.accept()
+            ot.accept()
+            nev += 1
+        dtc = time.perf_counter() - t0
+        cpu = {"value": dtc / nev * steps, "unit": "s per 10,000 steps (likelihood work)",
+               "cores": 1, "kind": "port",
+               "sample": f"{nev} full FP64 evaluations of the C1 workload ({dtc:.2f} s), "
+                         f"scaled to {steps}"}
+        log(f"[bench] C1 oracle: {cpu['value']:.2f} s per {steps} evaluations; final-state "
+            f"ln L rel err {err['final_state_rel']:.1e}")
+    print(json.dumps({
+        "metric": "seconds per 10,000 plain MCMC steps (16 taxa, HKY, 1,000 sites)",
+        "value": dt / steps * 10_000, "unit": "s", "n_gpus": 1, "steps": steps, "warmup": 200,
+        "ms_per_step": dt / steps * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Yule tree, sequences simulated along it)",
+        "config": {"workload": wl["name"], "taxa": L, "patterns": P, "sites": 1000,
+                   "model": "HKY kappa 2, equal frequencies", "categories": 1,
+                   "evaluation": "full (every node) at each step",
+                   "operators": "NodeSlide 4, TreeScale 1, NarrowExchange 2"},
+        "roofline": None, "cpu_baseline": cpu, "error_vs_fp64_oracle": err,
+        "chain_stats": {k: st[k] for k in ("steps", "accepted", "exact_evals")},
+    }), flush=True)
+
+
 def bench_batch(args):
     """256 independent HKY+Gamma4 instances (50-500 taxa, 2k-50k patterns),
     every instance fully evaluated in one persistent launch per step,
@@ -635,9 +704,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-f32", action="store_true", help="profiling runs: FP32 kernel only")
-    ap.add_argument("--workload", choices=["gtr128", "batch", "codon", "mcmc", "hetero",
+    ap.add_argument("--workload", choices=["gtr128", "c1", "batch", "codon", "mcmc", "hetero",
                                            "storage", "ingest"],
                     default="gtr128")
+    ap.add_argument("--c1-steps", type=int, default=10_000)
     ap.add_argument("--mcmc-taxa", type=int, default=200)
     ap.add_argument("--mcmc-sites", type=int, default=200_000)
     ap.add_argument("--mcmc-steps", type=int, default=100_000)
@@ -654,6 +724,8 @@ def main():
               "evaluation; that exchange is not built).", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "c1":
+        return bench_c1(args)
     if args.workload == "batch":
         return bench_batch(args)
     if args.workload == "codon":

@@ -95,6 +95,10 @@ typedef struct {
   uint32_t tune_every;
   uint64_t tune_cutoff;
   double tune_magnitude;
+  /* 1: every exact evaluation recomputes every node, as a run whose tree is
+   * marked whole before each likelihood call; 0 (default): only the changed
+   * edges' paths to the root (tree.rs:357-375) */
+  uint32_t full_evaluations;
 } b3m_chain_config;
 
 void b3m_chain_config_default(b3m_chain_config *cfg);

@@ -151,6 +151,32 @@ def _oracle_loglik(d, chain):
     return lik.likelihood(), par, h, a
 
 
+def test_chain_full_evaluations_hky_one_category(hip_device):
+    """The reference's CPU-runnable case in small: HKY (kappa 2, equal
+    frequencies) as fixed exchangeabilities, one category, the substitution
+    model never moved, every node recomputed at each step
+    (full_evaluations); after a few hundred steps the chain's ln L equals the
+    oracle's FP64 value of its state, with the pattern weights of a
+    compressed alignment."""
+    from aspartik_amd import workloads
+    wl = workloads.hky_cpu_mcmc(num_leaves=10, num_sites=400, seed=5)
+    L, P = wl["codes"].shape
+    c = Chain(wl["codes"], wl["parents"], wl["children"], wl["heights"], wl["gtr_rates"],
+              wl["freqs"], 1.0, weights=wl["weights"], categories=1, w_rates=0.0, w_alpha=0.0,
+              full_evaluations=1, seed=9)
+    for rep in range(3):
+        _, stats, _ = c.run(200)
+        assert stats["op_proposed"][2] == 0 and stats["op_proposed"][3] == 0
+        ll = c.state()[0]
+        par, ch, h, r, _ = c.params()
+        np.testing.assert_allclose(r / r.sum(), wl["gtr_rates"] / wl["gtr_rates"].sum())
+        t = O.Tree(L, par, ch, h)
+        ref = O.GammaLikelihood(t, O.Model.hky(wl["kappa"], wl["freqs"]), wl["codes"], P,
+                                wl["weights"], num_categories=1).likelihood()
+        assert abs(ll - ref) <= 1e-6 * abs(ref), (rep, ll, ref)
+    c.close()
+
+
 def test_chain_loglik_matches_oracle_after_many_moves(hip_device):
     d = _data(12, 600, 1)
     c = Chain(d["codes"], d["parents"], d["children"], d["heights"], d["rates"], d["freqs"],
