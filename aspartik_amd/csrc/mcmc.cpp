@@ -9,11 +9,14 @@
 // ranked tree up to a constant); flat Dirichlet on the exchangeabilities;
 // Exponential on alpha.  Operators as b3's Python ones (see b3mcmc.h).
 //
-// Delayed acceptance (Christen & Fox 2005): for continuous moves the first
-// stage accepts with min(1, exp(df + dprior + h)), df the surrogate's change
-// of ln L; only then is the exact likelihood evaluated and the move accepted
-// with min(1, exp(dlnL - df)).  Topology moves go straight to the exact MH
-// test.  Every stage is an MH test, so the chain keeps the exact posterior.
+// Delayed acceptance (Christen & Fox 2005): the first stage accepts with
+// min(1, exp(df + dprior + h)), df the surrogate's change of ln L; only then
+// is the exact likelihood evaluated and the move accepted with
+// min(1, exp(dlnL - df)).  Topology moves go straight to the exact MH test,
+// or, with screen_topology, pass the first stage too: the surrogate's
+// features (log edge lengths by node label, log rates, log alpha) are a
+// function of the whole labelled state, which is all the two-stage test
+// needs.  Every stage is an MH test, so the chain keeps the exact posterior.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -794,7 +797,7 @@ b3h_status b3m_chain_run(b3m_chain *c, uint64_t steps, uint32_t mode, uint32_t r
       const double prior = log_prior(c);
       if (prior == -INFINITY) {
         s.prior_rejects++;
-      } else if (mode == B3M_DELAYED && op != kNarrow) {
+      } else if (mode == B3M_DELAYED && (op != kNarrow || c->cfg.screen_topology)) {
         double fnew = 0.0;
         if ((st = surrogate(&fnew))) break;
         const double df = fnew - c->surrogate_cur;
@@ -828,7 +831,7 @@ b3h_status b3m_chain_run(b3m_chain *c, uint64_t steps, uint32_t mode, uint32_t r
         }
       }
     }
-    if (accept && mode == B3M_DELAYED && op == kNarrow) {
+    if (accept && mode == B3M_DELAYED && op == kNarrow && !c->cfg.screen_topology) {
       if ((st = surrogate(&c->surrogate_cur))) break;
     }
     finalize(c, accept);
@@ -854,7 +857,7 @@ b3h_status b3m_chain_run(b3m_chain *c, uint64_t steps, uint32_t mode, uint32_t r
       for (int i = 0; i < 6; i++) row[B3M_COL_RATE0 + i] = c->rates[i];
     }
   }
-  // a surrogate that was not refreshed after a topology change is stale
+  // a plain run moves the state without the surrogate: its value is stale
   if (mode != B3M_DELAYED) c->surrogate_valid = false;
   s.seconds = now() - t_start;
   s.tuning[kNodeSlide] = s.tuning[kNarrow] = -1.0;

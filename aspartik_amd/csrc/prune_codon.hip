@@ -286,13 +286,27 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
           r[1] *= kLoScale;
         }
       }
+      // issue order of the row block: two A-fragment reads ahead, then one
+      // read per MFMA, so that no MFMA waits out a whole LDS round trip
+      __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+#pragma unroll
+      for (uint32_t k = 0; k < 12; k++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
 #pragma unroll
       for (uint32_t side = 0; side < 2; side++) {
         if ((srcs[side] & SRC_TAG) == SRC_TIP && __builtin_amdgcn_ballot_w64(code[side] >= S)) {
           const bool unk = code[side] >= S;
+          // opaque here, so that the 16 selects are made in this (rare)
+          // branch instead of being hoisted out of the node loop as 32
+          // registers of constants held for the whole kernel
+          int l = lim;
+          asm volatile("" : "+v"(l));
 #pragma unroll
           for (uint32_t q = 0; q < 16; q++)
-            if (unk) r[side][q] = (int)row_of(0, 16 * rb + q) < lim ? 1.0f : 0.0f;
+            if (unk) r[side][q] = (int)row_of(0, 16 * rb + q) < l ? 1.0f : 0.0f;
         }
       }
       acc[rb] = r[0] * r[1];

@@ -12,6 +12,7 @@ to the root), the error against the FP64 CPU oracle, and the oracle's own
 throughput on the host cores (cpu_baseline).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --workload c1                        (16-taxon HKY chain, 10,000 steps)
        python bench.py --workload batch [--instances 256]   (batched HKY+Gamma4)
        python bench.py --workload codon                     (GY94 61-state, MFMA)
        python bench.py --workload mcmc                      (ESS/s, delayed acceptance)
@@ -338,14 +339,45 @@ def bench_codon(args):
     }), flush=True)
 
 
+def _run_chunked(chain, steps, mode, name, chunk=50_000, max_rows=100_000):
+    """Run a chain in chunks of ``chunk`` steps on the same handle (the chain
+    state carries over, so the steps are those of one run), logging each
+    chunk; returns the concatenated trace, recorded every
+    ceil(steps / max_rows) steps, and the summed statistics."""
+    every = -(-steps // max_rows)
+    chunk = max(every, chunk // every * every)
+    traces, total = [], None
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        tr, st, _ = chain.run(n, mode=mode, record_every=every)
+        traces.append(tr)
+        if total is None:
+            total = st
+        else:
+            for k, v in st.items():
+                if k == "tuning":
+                    total[k] = v
+                elif isinstance(v, list):
+                    total[k] = [a + b for a, b in zip(total[k], v)]
+                else:
+                    total[k] += v
+        done += n
+        log(f"[bench] {name}: {done} of {steps} steps, {total['seconds']:.1f} s")
+    return np.concatenate(traces), total
+
+
 def bench_mcmc(args):
     """ESS per second of delayed-acceptance MH against plain MH: GTR+Gamma4,
     200 taxa, 200,000 sites.  Burn-in with plain MH collects (features,
     exact ln L) pairs; the MLP surrogate (404 -> 256 -> 256 -> 1) is trained
     on them; then plain and delayed-acceptance chains run the same number of
-    steps from the burn-in's final state.  ESS is Geyer's estimator on the
-    log posterior, tree length and alpha traces; ESS/s uses each chain's
-    wall clock (likelihood, surrogate and host work)."""
+    steps from the burn-in's final state (with --screen-topology 1, the
+    default, NarrowExchange passes the surrogate's first stage too).  ESS is
+    Geyer's estimator on the
+    log posterior, tree length and alpha traces (recorded every
+    ceil(steps / 100,000) steps); ESS/s uses each chain's wall clock
+    (likelihood, surrogate and host work)."""
     from aspartik_amd import workloads
     from aspartik_amd.b3 import Chain, Surrogate, ess
 
@@ -393,10 +425,11 @@ def bench_mcmc(args):
     par, ch, h, r, a = state
     res = {}
     for name, mode, seed in [("plain", "plain", 202), ("delayed", "delayed", 303)]:
-        c = Chain(wl["codes"], par, ch, h, r, wl["freqs"], a, seed=seed, **kw)
+        c = Chain(wl["codes"], par, ch, h, r, wl["freqs"], a, seed=seed,
+                  screen_topology=args.screen_topology, **kw)
         if mode == "delayed":
             c.set_surrogate(sur)
-        trace, st, _ = c.run(args.mcmc_steps, mode=mode)
+        trace, st = _run_chunked(c, args.mcmc_steps, mode, name)
         c.close()
         e = {k: ess(trace[:, col]) for k, col in (("posterior", 1), ("tree_length", 5),
                                                    ("alpha", 6))}
@@ -446,6 +479,9 @@ def bench_mcmc(args):
         "config": {"workload": wl["name"], "taxa": L, "patterns": P, "categories": 4,
                    "model": "GTR+Gamma4 alpha 0.5", "surrogate": f"{nf}->256->256->1 GELU BF16",
                    "burnin_steps": args.burnin, "chain_steps": args.mcmc_steps,
+                   "trace_every": -(-args.mcmc_steps // 100_000),
+                   "topology_moves": "surrogate stage, then exact" if args.screen_topology
+                                     else "exact MH test only",
                    "operators": "NodeSlide 4, TreeScale 1, DeltaExchange 2, ScaleReal(alpha) 1, "
                                 "NarrowExchange 2 (b3's defaults, tuned in burn-in)"},
         "ess_per_s_plain": res["plain"]["ess_per_s"], "ess_per_s_delayed": res["delayed"]["ess_per_s"],
@@ -714,6 +750,8 @@ def main():
     ap.add_argument("--burnin", type=int, default=30_000)
     ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--clip", type=float, default=100.0)
+    ap.add_argument("--screen-topology", type=int, choices=[0, 1], default=1,
+                    help="delayed acceptance: topology moves pass the surrogate stage too")
     ap.add_argument("--codon-taxa", type=int, default=64)
     ap.add_argument("--codon-patterns", type=int, default=20_000)
     ap.add_argument("--instances", type=int, default=256)

@@ -99,6 +99,11 @@ typedef struct {
    * marked whole before each likelihood call; 0 (default): only the changed
    * edges' paths to the root (tree.rs:357-375) */
   uint32_t full_evaluations;
+  /* delayed acceptance: 1: topology moves (NarrowExchange) pass the
+   * surrogate's first stage like every other operator (the features are a
+   * function of the whole labelled state); 0 (default): they go straight to
+   * the exact MH test */
+  uint32_t screen_topology;
 } b3m_chain_config;
 
 void b3m_chain_config_default(b3m_chain_config *cfg);
@@ -122,7 +127,7 @@ b3h_status b3m_chain_features(const b3m_chain *c, float *out);
 
 enum {
   B3M_PLAIN = 0,    /* exact likelihood every step */
-  B3M_DELAYED = 1   /* surrogate first stage (continuous moves), exact second */
+  B3M_DELAYED = 1   /* surrogate first stage, exact second (screen_topology) */
 };
 
 /* Trace columns, one row per recorded step */

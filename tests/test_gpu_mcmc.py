@@ -8,8 +8,9 @@
   model restores, topology moves) against the FP64 oracle for the chain's
   current state, and its log prior against the formula.
 * Delayed acceptance keeps the posterior: posterior means of a plain chain
-  and of delayed-acceptance chains (trained and untrained surrogate) agree
-  within 5 Monte Carlo standard errors (ESS-based).
+  and of delayed-acceptance chains (trained and untrained surrogate;
+  topology moves straight to the exact test or screened by the surrogate)
+  agree within 5 Monte Carlo standard errors (ESS-based).
 """
 import math
 
@@ -215,18 +216,28 @@ def test_delayed_acceptance_keeps_the_posterior(hip_device):
     untrained.fit_scaling(X, y)
     cols = {"tree_length": 5, "alpha": 6, "root_height": 4}
     runs = {}
-    for name, sur, mode, seed in [("plain", None, "plain", 21), ("delayed", s, "delayed", 22),
-                                  ("untrained", untrained, "delayed", 23)]:
-        c = Chain(d["codes"], par, ch, h, r, d["freqs"], a, seed=seed, **kw)
+    for name, sur, mode, seed, screen in [("plain", None, "plain", 21, 0),
+                                          ("delayed", s, "delayed", 22, 0),
+                                          ("untrained", untrained, "delayed", 23, 0),
+                                          ("screened", s, "delayed", 24, 1)]:
+        c = Chain(d["codes"], par, ch, h, r, d["freqs"], a, seed=seed, screen_topology=screen,
+                  **kw)
         if sur is not None:
             c.set_surrogate(sur)
         trace, stats, _ = c.run(20000, mode=mode)
         runs[name] = (trace, stats)
     assert runs["delayed"][1]["stage1_rejects"] > 0
     assert runs["delayed"][1]["exact_evals"] < runs["plain"][1]["exact_evals"]
+    # with screen_topology every proposal that is neither aborted nor
+    # prior-rejected is screened (one surrogate call each, plus one for the
+    # starting state); without it NarrowExchange skips the surrogate
+    st = runs["screened"][1]
+    assert st["surrogate_evals"] == st["steps"] - st["aborts"] - st["prior_rejects"] + 1
+    sd = runs["delayed"][1]
+    assert sd["surrogate_evals"] < sd["steps"] - sd["aborts"] - sd["prior_rejects"]
     for col_name, col in cols.items():
         m0, s0 = _mean_and_se(runs["plain"][0][:, col])
-        for other in ("delayed", "untrained"):
+        for other in ("delayed", "untrained", "screened"):
             m1, s1 = _mean_and_se(runs[other][0][:, col])
             z = abs(m0 - m1) / math.hypot(s0, s1)
             assert z < 5.0, (col_name, other, m0, m1, s0, s1)
