@@ -93,6 +93,7 @@ class ChainConfig(C.Structure):
         ("tune_cutoff", C.c_uint64),
         ("tune_magnitude", C.c_double),
         ("full_evaluations", C.c_uint32), ("screen_topology", C.c_uint32),
+        ("surrogate_beta", C.c_double),
     ]
 
 

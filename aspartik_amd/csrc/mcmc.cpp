@@ -10,8 +10,8 @@
 // Exponential on alpha.  Operators as b3's Python ones (see b3mcmc.h).
 //
 // Delayed acceptance (Christen & Fox 2005): the first stage accepts with
-// min(1, exp(df + dprior + h)), df the surrogate's change of ln L; only then
-// is the exact likelihood evaluated and the move accepted with
+// min(1, exp(df + dprior + h)), df = beta x the surrogate's change of ln L;
+// only then is the exact likelihood evaluated and the move accepted with
 // min(1, exp(dlnL - df)).  Topology moves go straight to the exact MH test,
 // or, with screen_topology, pass the first stage too: the surrogate's
 // features (log edge lengths by node label, log rates, log alpha) are a
@@ -512,6 +512,7 @@ void b3m_chain_config_default(b3m_chain_config *cfg) {
   cfg->tune_every = 0;
   cfg->tune_cutoff = 0;
   cfg->tune_magnitude = 0.5;
+  cfg->surrogate_beta = 1.0;
 }
 
 b3h_status b3m_chain_create(const b3m_chain_config *cfg, const uint8_t *codes, uint32_t L,
@@ -527,7 +528,8 @@ b3h_status b3m_chain_create(const b3m_chain_config *cfg, const uint8_t *codes, u
   if (!(cfg->birth_rate > 0.0) || !(cfg->alpha_prior_rate > 0.0) || !(alpha > 0.0) ||
       !(cfg->scale_factor > 0.0 && cfg->scale_factor < 1.0) ||
       !(cfg->alpha_factor >= 0.0 && cfg->alpha_factor < 1.0) ||
-      !(cfg->delta_tuning >= 0.0 && cfg->delta_tuning < 1.0))
+      !(cfg->delta_tuning >= 0.0 && cfg->delta_tuning < 1.0) ||
+      !(cfg->surrogate_beta > 0.0 && std::isfinite(cfg->surrogate_beta)))
     return fail(B3H_ERR_ARG, "bad prior or operator setting");
   const double wsum =
       cfg->w_node_slide + cfg->w_tree_scale + cfg->w_rates + cfg->w_alpha + cfg->w_narrow;
@@ -800,7 +802,7 @@ b3h_status b3m_chain_run(b3m_chain *c, uint64_t steps, uint32_t mode, uint32_t r
       } else if (mode == B3M_DELAYED && (op != kNarrow || c->cfg.screen_topology)) {
         double fnew = 0.0;
         if ((st = surrogate(&fnew))) break;
-        const double df = fnew - c->surrogate_cur;
+        const double df = c->cfg.surrogate_beta * (fnew - c->surrogate_cur);
         const double r1 = df + (prior - c->logprior) + hastings;
         if (!(r1 > std::log(c->rng.uniform()))) {
           s.stage1_rejects++;

@@ -373,11 +373,12 @@ def bench_mcmc(args):
     exact ln L) pairs; the MLP surrogate (404 -> 256 -> 256 -> 1) is trained
     on them; then plain and delayed-acceptance chains run the same number of
     steps from the burn-in's final state (with --screen-topology 1, the
-    default, NarrowExchange passes the surrogate's first stage too).  ESS is
-    Geyer's estimator on the
-    log posterior, tree length and alpha traces (recorded every
-    ceil(steps / 100,000) steps); ESS/s uses each chain's wall clock
-    (likelihood, surrogate and host work)."""
+    default, NarrowExchange passes the surrogate's first stage too; one
+    delayed chain per first-stage temperature in --betas, the first being
+    the headline).  ESS is Geyer's estimator on the log posterior, tree
+    length and alpha traces (recorded every ceil(steps / 100,000) steps);
+    ESS/s uses each chain's wall clock (likelihood, surrogate and host
+    work)."""
     from aspartik_amd import workloads
     from aspartik_amd.b3 import Chain, Surrogate, ess
 
@@ -391,9 +392,12 @@ def bench_mcmc(args):
                  wl["freqs"], wl["alpha"], seed=101, tune_every=max(50, args.burnin // 40),
                  tune_cutoff=args.burnin, **kw)
     t0 = time.perf_counter()
-    _, bstats, (X, y) = burn.run(args.burnin, collect=args.burnin)
+    collect = args.collect if 0 < args.collect < args.burnin else args.burnin
+    if collect < args.burnin:   # training pairs from the last `collect` steps only
+        burn.run(args.burnin - collect)
+    _, bstats, (X, y) = burn.run(collect, collect=collect)
     log(f"[bench] burn-in {args.burnin} steps: {time.perf_counter() - t0:.1f} s, "
-        f"{len(y)} exact evaluations collected")
+        f"{len(y)} exact evaluations collected from the last {collect}")
     state = burn.params()
     nf = burn.num_features
     tuned = dict(scale_factor=bstats["tuning"][1], delta_tuning=bstats["tuning"][2],
@@ -424,9 +428,13 @@ def bench_mcmc(args):
         f"(ln L std {held['loglik_std']:.2f})")
     par, ch, h, r, a = state
     res = {}
-    for name, mode, seed in [("plain", "plain", 202), ("delayed", "delayed", 303)]:
+    betas = [float(b) for b in args.betas.split(",")]
+    chains = [("plain", "plain", args.chain_seed, 1.0)] + [
+        ("delayed" if i == 0 else f"delayed_beta{b:g}", "delayed", args.chain_seed + 1 + i, b)
+        for i, b in enumerate(betas)]
+    for name, mode, seed, beta in chains:
         c = Chain(wl["codes"], par, ch, h, r, wl["freqs"], a, seed=seed,
-                  screen_topology=args.screen_topology, **kw)
+                  screen_topology=args.screen_topology, surrogate_beta=beta, **kw)
         if mode == "delayed":
             c.set_surrogate(sur)
         trace, st = _run_chunked(c, args.mcmc_steps, mode, name)
@@ -480,6 +488,8 @@ def bench_mcmc(args):
                    "model": "GTR+Gamma4 alpha 0.5", "surrogate": f"{nf}->256->256->1 GELU BF16",
                    "burnin_steps": args.burnin, "chain_steps": args.mcmc_steps,
                    "trace_every": -(-args.mcmc_steps // 100_000),
+                   "training_pairs_from_last_steps": collect, "surrogate_betas": betas,
+                   "chain_seed": args.chain_seed,
                    "topology_moves": "surrogate stage, then exact" if args.screen_topology
                                      else "exact MH test only",
                    "operators": "NodeSlide 4, TreeScale 1, DeltaExchange 2, ScaleReal(alpha) 1, "
@@ -488,6 +498,8 @@ def bench_mcmc(args):
         "ess_ratio_delayed_vs_plain": ratio,
         "ess_ratio_including_training": ratio_incl_training,
         "plain": res["plain"], "delayed": res["delayed"], "surrogate_held_out": held,
+        "other_betas": {n: dict(v, ess_ratio_vs_plain=v["ess_per_s"] / res["plain"]["ess_per_s"])
+                        for n, v in res.items() if n not in ("plain", "delayed")},
         "burnin_stats": bstats, "cpu_baseline": cpu, "roofline": None,
     }), flush=True)
 
@@ -747,9 +759,16 @@ def main():
     ap.add_argument("--mcmc-taxa", type=int, default=200)
     ap.add_argument("--mcmc-sites", type=int, default=200_000)
     ap.add_argument("--mcmc-steps", type=int, default=100_000)
-    ap.add_argument("--burnin", type=int, default=30_000)
+    ap.add_argument("--burnin", type=int, default=60_000)
     ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--clip", type=float, default=100.0)
+    ap.add_argument("--collect", type=int, default=30_000,
+                    help="surrogate training pairs from the last N burn-in steps (0: all)")
+    ap.add_argument("--betas", default="0.5",
+                    help="delayed acceptance: first-stage temperatures, one chain each; "
+                         "the first is the headline")
+    ap.add_argument("--chain-seed", type=int, default=202,
+                    help="seed of the plain chain; delayed chains take the next ones")
     ap.add_argument("--screen-topology", type=int, choices=[0, 1], default=1,
                     help="delayed acceptance: topology moves pass the surrogate stage too")
     ap.add_argument("--codon-taxa", type=int, default=64)

@@ -104,6 +104,11 @@ typedef struct {
    * function of the whole labelled state); 0 (default): they go straight to
    * the exact MH test */
   uint32_t screen_topology;
+  /* delayed acceptance: the first stage targets exp(beta f) x prior, f the
+   * surrogate's ln L; the second accepts with min(1, exp(dlnL - beta df)).
+   * beta < 1 lets more moves reach the exact test when f is noisy; any
+   * beta > 0 keeps the exact posterior (default 1) */
+  double surrogate_beta;
 } b3m_chain_config;
 
 void b3m_chain_config_default(b3m_chain_config *cfg);

@@ -9,7 +9,8 @@
   current state, and its log prior against the formula.
 * Delayed acceptance keeps the posterior: posterior means of a plain chain
   and of delayed-acceptance chains (trained and untrained surrogate;
-  topology moves straight to the exact test or screened by the surrogate)
+  topology moves straight to the exact test or screened by the surrogate,
+  first stage tempered)
   agree within 5 Monte Carlo standard errors (ESS-based).
 """
 import math
@@ -216,12 +217,13 @@ def test_delayed_acceptance_keeps_the_posterior(hip_device):
     untrained.fit_scaling(X, y)
     cols = {"tree_length": 5, "alpha": 6, "root_height": 4}
     runs = {}
-    for name, sur, mode, seed, screen in [("plain", None, "plain", 21, 0),
-                                          ("delayed", s, "delayed", 22, 0),
-                                          ("untrained", untrained, "delayed", 23, 0),
-                                          ("screened", s, "delayed", 24, 1)]:
+    # "screened": topology moves through the first stage, tempered at beta 0.5
+    for name, sur, mode, seed, screen, beta in [("plain", None, "plain", 21, 0, 1.0),
+                                                ("delayed", s, "delayed", 22, 0, 1.0),
+                                                ("untrained", untrained, "delayed", 23, 0, 1.0),
+                                                ("screened", s, "delayed", 24, 1, 0.5)]:
         c = Chain(d["codes"], par, ch, h, r, d["freqs"], a, seed=seed, screen_topology=screen,
-                  **kw)
+                  surrogate_beta=beta, **kw)
         if sur is not None:
             c.set_surrogate(sur)
         trace, stats, _ = c.run(20000, mode=mode)
