@@ -80,7 +80,7 @@ struct b3h_calc {
   // staging (pinned host -> device), double buffered
   size_t stage_bytes = 0;
   unsigned char *h_stage[kStages] = {nullptr, nullptr};
-  double *h_total = nullptr;  // pinned target of the total's read-back
+  double *h_total = nullptr;  // pinned host mirrors of the result slots' totals (kernel-written)
   unsigned char *d_stage[kStages] = {nullptr, nullptr};
   hipEvent_t stage_ev[kStages] = {nullptr, nullptr};
   int stage_next = 0;
@@ -395,6 +395,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     m.blocksum = c->d_mix_blocksum;
     m.counter = c->d_counter;
     m.total = r_total;
+    m.host_total = c->h_total + ws;
     m.ln_inv_k = std::log(1.0 / (double)K);
     m.P = c->P;
     m.K = K;
@@ -416,6 +417,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     a.blocksum = c->d_blocksum;
     a.counter = c->d_counter;
     a.total = r_total;
+    a.host_total = c->h_total + ws;
     a.pi0 = freqs[0];
     a.pi1 = freqs[1];
     a.pi2 = freqs[2];
@@ -479,6 +481,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     m.blocksum = c->d_mix_blocksum;
     m.counter = c->d_counter;
     m.total = r_total;
+    m.host_total = c->h_total + ws;
     m.ln_inv_k = std::log(1.0 / (double)K);
     m.P = c->P;
     m.K = K;
@@ -617,6 +620,7 @@ b3h_status run_propose_prop(b3h_calc *c, uint32_t nn, const uint32_t *nodes, uin
   m.blocksum = c->d_mix_blocksum;
   m.counter = c->d_counter;
   m.total = c->d_total + ws;
+  m.host_total = c->h_total + ws;
   m.ln_inv_k = std::log(1.0 / (double)K);
   m.P = c->P;
   m.K = K;
@@ -1036,6 +1040,7 @@ b3h_status b3h_set_classes(b3h_calc *c, const uint32_t *classes) {
   m.blocksum = c->d_mix_blocksum;
   m.counter = c->d_counter;
   m.total = c->d_total + slot;
+  m.host_total = c->h_total + slot;
   m.ln_inv_k = std::log(1.0 / (double)c->K);
   m.P = c->P;
   m.K = c->K;
@@ -1121,11 +1126,10 @@ b3h_status b3h_total(b3h_calc *c, double *out) {
   uint32_t slot;
   if ((st = result_slot(c, &slot))) return st;
   if ((st = set_device(c))) return st;
-  // into pinned memory: a pageable destination goes through HIP's own
-  // staging copy
-  HIPCHK(hipMemcpyAsync(c->h_total, c->d_total + slot, 8, hipMemcpyDeviceToHost, c->stream));
+  // every kernel that writes a slot's total also writes it to the slot's
+  // pinned host mirror: wait for the stream, no copy
   HIPCHK(hipStreamSynchronize(c->stream));
-  *out = *c->h_total;
+  *out = *(volatile double *)(c->h_total + slot);
   return B3H_OK;
 }
 

@@ -47,6 +47,7 @@ struct F32Args {
   double *blocksum;       // [ntiles] per-tile partial sums
   uint32_t *counter;      // last-tile ticket (self-resetting)
   double *total;
+  double *host_total;     // pinned host mirror of *total, or null
   double pi0, pi1, pi2, pi3;
   const double *pik;        // [K][4] root frequencies per category, or null (pi0..pi3)
   const uint8_t *classes;   // [P] category selected per pattern, or null (mean over categories)
@@ -156,6 +157,7 @@ struct MixArgs {
   double *blocksum;    // [mix_blocks(P)]
   uint32_t *counter;   // last-block ticket (self-resetting)
   double *total;
+  double *host_total;     // pinned host mirror of *total, or null
   double ln_inv_k;
   uint32_t P, K;
 };

@@ -27,8 +27,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 // pruning itself at thousands of tiles.  rocPRIM's look-back scan state uses
 // the same order-only idiom.
 __device__ __forceinline__ void last_block_total(double block_value, double *blocksum,
-                                                 uint32_t *counter, double *total, int *flag,
-                                                 uint32_t index, uint32_t count) {
+                                                 uint32_t *counter, double *total,
+                                                 double *host_total, int *flag, uint32_t index,
+                                                 uint32_t count) {
   if (threadIdx.x == 0) {
     __hip_atomic_store(&blocksum[index], block_value, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
@@ -48,6 +49,9 @@ __device__ __forceinline__ void last_block_total(double block_value, double *blo
     acc = wave_sum(acc);
     if (threadIdx.x == 0) {
       *total = acc;
+      // pinned host memory: visible to the host once the kernel has
+      // completed (after a stream synchronisation), with no copy
+      if (host_total) *host_total = acc;
       __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
