@@ -331,8 +331,14 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
       for (uint32_t j = 0; j < 4; j++) hpi[4 * k + j] = c->freqs_k[k][j];
   if (path && !psrc.empty()) std::memcpy(h + off_src, psrc.data(), psrc.size() * 4);
   unsigned char *d = c->d_stage[s];
-  HIPCHK(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipEventRecord(c->stage_ev[s], c->stream));
+  // 4-state FP32: the P(t) build reads its inputs from the pinned staging
+  // buffer and writes the device copy that the pruning kernel reads, which
+  // saves a copy launch per evaluation; the other paths copy first
+  const bool fused_upload = f32 && !codon;
+  if (!fused_upload) {
+    HIPCHK(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->stage_ev[s], c->stream));
+  }
   const Entry *d_entries = reinterpret_cast<const Entry *>(d);
   const double *d_mats = reinterpret_cast<const double *>(d + off_m);
   const double *d_times = reinterpret_cast<const double *>(d + off_t);
@@ -401,9 +407,13 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     m.K = K;
     HIPCHK(launch_mix(m, c->stream));
   } else if (f32) {
-    HIPCHK(launch_pbuild<float>(d_entries, n, K, S, mode, d_mats, d_times,
-                                d_rates, c->d_model, mstride,
-                                static_cast<float *>(c->d_pmats), c->stream));
+    HIPCHK(launch_pbuild<float>(reinterpret_cast<const Entry *>(h), n, K, S, mode,
+                                reinterpret_cast<const double *>(h + off_m),
+                                reinterpret_cast<const double *>(h + off_t),
+                                reinterpret_cast<const double *>(h + off_r), c->d_model, mstride,
+                                static_cast<float *>(c->d_pmats), c->stream, h, d,
+                                (uint32_t)((total + 15) / 16)));
+    HIPCHK(hipEventRecord(c->stage_ev[s], c->stream));  // the last reader of h
     F32Args a{};
     a.entries = d_entries;
     a.pmats = static_cast<const float *>(c->d_pmats);

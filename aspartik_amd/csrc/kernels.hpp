@@ -12,12 +12,16 @@ hipError_t launch_eigen(DevModel *m, hipStream_t s);
 //   mode 1: P(times[m] * rates[c]) from the device eigensystem
 //   mode 2: P(times[c * 2n + m]), branch lengths given per category
 // model_stride 0: one model for every category; 1: model[c] per category.
+// copy_units > 0: the launch also copies copy_units 16-byte units from
+// copy_src (pinned host memory) to copy_dst, the upload of a staging buffer
+// whose inputs the build then reads in place.
 template <typename T>
 hipError_t launch_pbuild(const Entry *entries, uint32_t n, uint32_t K,
                          uint32_t S, int mode, const double *mats,
                          const double *times, const double *rates,
                          const DevModel *model, uint32_t model_stride, T *out,
-                         hipStream_t s);
+                         hipStream_t s, const void *copy_src = nullptr,
+                         void *copy_dst = nullptr, uint32_t copy_units = 0);
 
 struct PbuildDesc {
   const Entry *entries;

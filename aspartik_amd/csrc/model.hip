@@ -169,7 +169,14 @@ template <typename T>
 __global__ void pbuild_kernel(const Entry *entries, uint32_t n, uint32_t K,
                               uint32_t S, int mode, const double *mats,
                               const double *times, const double *rates,
-                              const DevModel *model, uint32_t model_stride, T *out) {
+                              const DevModel *model, uint32_t model_stride, T *out,
+                              const uint4 *copy_src, uint4 *copy_dst, uint32_t copy_units) {
+  // the upload of the staging buffer, when this launch carries it: the
+  // inputs above then point into the pinned host staging buffer itself, and
+  // the kernels after this one read the device copy
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < copy_units;
+       u += gridDim.x * blockDim.x)
+    copy_dst[u] = copy_src[u];
   const uint32_t SS = S * S;
   const uint64_t total = (uint64_t)n * 2 * K * SS;
   for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
@@ -275,13 +282,16 @@ hipError_t launch_pbuild(const Entry *entries, uint32_t n, uint32_t K,
                          uint32_t S, int mode, const double *mats,
                          const double *times, const double *rates,
                          const DevModel *model, uint32_t model_stride, T *out,
-                         hipStream_t s) {
+                         hipStream_t s, const void *copy_src, void *copy_dst,
+                         uint32_t copy_units) {
   const uint64_t total = (uint64_t)n * 2 * K * S * S;
-  uint64_t blocks = (total + 255) / 256;
+  uint64_t blocks = std::max<uint64_t>((total + 255) / 256, (copy_units + 255u) / 256u);
   if (blocks > 4096) blocks = 4096;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(pbuild_kernel<T>, dim3((uint32_t)blocks), dim3(256), 0, s,
-                     entries, n, K, S, mode, mats, times, rates, model, model_stride, out);
+                     entries, n, K, S, mode, mats, times, rates, model, model_stride, out,
+                     static_cast<const uint4 *>(copy_src), static_cast<uint4 *>(copy_dst),
+                     copy_units);
   return hipGetLastError();
 }
 
@@ -289,11 +299,11 @@ template hipError_t launch_pbuild<float>(const Entry *, uint32_t, uint32_t,
                                          uint32_t, int, const double *,
                                          const double *, const double *,
                                          const DevModel *, uint32_t, float *,
-                                         hipStream_t);
+                                         hipStream_t, const void *, void *, uint32_t);
 template hipError_t launch_pbuild<double>(const Entry *, uint32_t, uint32_t,
                                           uint32_t, int, const double *,
                                           const double *, const double *,
                                           const DevModel *, uint32_t, double *,
-                                          hipStream_t);
+                                          hipStream_t, const void *, void *, uint32_t);
 
 }  // namespace b3h
