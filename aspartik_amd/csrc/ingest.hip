@@ -191,12 +191,26 @@ b3h_status dedup_host(const uint8_t *msa, uint32_t S, uint64_t N, uint8_t *leave
 
 // ---- device deduplication ------------------------------------------------
 
+// With col non-null the pass also writes the site-major copy of the
+// alignment that k_verify reads: column j as Sp = S rounded up to 16 bytes at
+// col + j * Sp, zero-padded, in 16-byte stores.
 __global__ void k_col_hash(const uint8_t *__restrict__ msa, uint32_t S, uint64_t N, uint64_t seed,
-                           uint64_t *__restrict__ h) {
+                           uint64_t *__restrict__ h, uint8_t *__restrict__ col, uint32_t Sp) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
   uint64_t x = (0x9E3779B97F4A7C15ull + seed) ^ S;
-  for (uint32_t s = 0; s < S; s++) x = (x ^ msa[(size_t)s * N + j]) * 0x100000001B3ull;
+  for (uint32_t c = 0; c < Sp; c += 16) {
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (uint32_t q = 0; q < 16; q++) {
+      if (c + q < S) {
+        const uint32_t b = msa[(size_t)(c + q) * N + j];
+        x = (x ^ b) * 0x100000001B3ull;
+        w[q >> 2] |= b << (8 * (q & 3));
+      }
+    }
+    if (col) *reinterpret_cast<uint4 *>(col + j * Sp + c) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
   x ^= x >> 30;
   x *= 0xbf58476d1ce4e5b9ull;
   x ^= x >> 27;
@@ -227,8 +241,10 @@ __global__ void k_insert(const uint64_t *__restrict__ h, uint64_t N,
 
 // Each site against the first site of its slot: equal bytes, or a hash
 // collision between different columns (then the caller hashes again with
-// another seed).  Counts per slot give the weights.
-__global__ void k_verify(const uint8_t *__restrict__ msa, uint32_t S, uint64_t N,
+// another seed).  Counts per slot give the weights.  The columns are compared
+// in the site-major copy (Sp bytes each, contiguous), so the representative's
+// column is read in 16-byte loads rather than one byte per row.
+__global__ void k_verify(const uint8_t *__restrict__ col, uint32_t Sp, uint64_t N,
                          const uint32_t *__restrict__ slot_of, const uint32_t *__restrict__ first,
                          uint32_t *__restrict__ count, uint32_t *__restrict__ is_first,
                          uint32_t *__restrict__ rep, uint32_t *__restrict__ collided) {
@@ -239,11 +255,15 @@ __global__ void k_verify(const uint8_t *__restrict__ msa, uint32_t S, uint64_t N
   rep[j] = r;
   is_first[j] = (r == j) ? 1u : 0u;
   if (r != j) {
-    for (uint32_t s = 0; s < S; s++)
-      if (msa[(size_t)s * N + j] != msa[(size_t)s * N + r]) {
+    const uint4 *a = reinterpret_cast<const uint4 *>(col + j * Sp);
+    const uint4 *b = reinterpret_cast<const uint4 *>(col + (uint64_t)r * Sp);
+    for (uint32_t c = 0; c < Sp / 16u; c++) {
+      const uint4 u = a[c], v = b[c];
+      if (u.x != v.x || u.y != v.y || u.z != v.z || u.w != v.w) {
         atomicOr(collided, 1u);
         break;
       }
+    }
   }
   atomicAdd(&count[slot], 1u);
 }
@@ -366,11 +386,13 @@ b3h_status dedup_device(int device, const uint8_t *msa, uint32_t S, uint64_t N, 
   DevBufs buf;
   uint64_t cap = 16;
   while (cap < 2 * N) cap <<= 1;
-  uint8_t *d_msa, *d_leaves;
+  uint8_t *d_msa, *d_leaves, *d_col;
+  const uint32_t Sp = (S + 15u) & ~15u;
   uint64_t *d_h, *d_s2p;
   unsigned long long *d_keys;
   uint32_t *d_first, *d_slot, *d_count, *d_isf, *d_pid, *d_rep, *d_w, *d_coll;
   DCHK(buf.alloc(&d_msa, (size_t)S * N));
+  DCHK(buf.alloc(&d_col, (size_t)Sp * N));
   DCHK(buf.alloc(&d_h, N * 8));
   DCHK(buf.alloc(&d_keys, cap * 8));
   DCHK(buf.alloc(&d_first, cap * 4));
@@ -388,11 +410,13 @@ b3h_status dedup_device(int device, const uint8_t *msa, uint32_t S, uint64_t N, 
     DCHK(hipMemsetAsync(d_first, 0xFF, cap * 4, stream));
     DCHK(hipMemsetAsync(d_count, 0, cap * 4, stream));
     DCHK(hipMemsetAsync(d_coll, 0, 4, stream));
-    hipLaunchKernelGGL(k_col_hash, grid, block, 0, stream, d_msa, S, N, seed, d_h);
+    // the site-major copy does not depend on the seed: written on the first pass
+    hipLaunchKernelGGL(k_col_hash, grid, block, 0, stream, d_msa, S, N, seed, d_h,
+                       seed == 0 ? d_col : nullptr, Sp);
     DCHK(hipGetLastError());
     hipLaunchKernelGGL(k_insert, grid, block, 0, stream, d_h, N, d_keys, d_first, cap - 1, d_slot);
     DCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_verify, grid, block, 0, stream, d_msa, S, N, d_slot, d_first, d_count,
+    hipLaunchKernelGGL(k_verify, grid, block, 0, stream, d_col, Sp, N, d_slot, d_first, d_count,
                        d_isf, d_rep, d_coll);
     DCHK(hipGetLastError());
     uint32_t coll = 0;
