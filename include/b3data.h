@@ -63,7 +63,8 @@ b3h_status b3d_msa_base_frequencies(const b3d_msa *msa, double *out4);
  *   weights[num_patterns], sites_to_patterns[num_sites]
  * *num_patterns receives the pattern count.  On a device the call holds
  * about 2 x num_sequences x num_sites bytes of device memory (the alignment
- * and a site-major copy) plus about 40 bytes per site of tables. */
+ * and a site-major copy) plus 64-96 bytes per site of hash table and
+ * indices. */
 b3h_status b3d_deduplicate(const uint8_t *msa, uint32_t num_sequences, uint64_t num_sites,
                            int device, uint8_t *leaves, uint64_t leaves_stride,
                            uint32_t *weights, uint64_t *sites_to_patterns,
