@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <chrono>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -81,6 +82,8 @@ struct b3h_calc {
   size_t stage_bytes = 0;
   unsigned char *h_stage[kStages] = {nullptr, nullptr};
   double *h_total = nullptr;  // pinned host mirrors of the result slots' totals (kernel-written)
+  uint32_t *h_flag = nullptr;  // per slot: sequence number stored after the mirror
+  uint32_t seq = 0, expect[2] = {0u, 0u};  // last number issued; expected per slot
   unsigned char *d_stage[kStages] = {nullptr, nullptr};
   hipEvent_t stage_ev[kStages] = {nullptr, nullptr};
   int stage_next = 0;
@@ -137,6 +140,7 @@ void release(b3h_calc *c) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_total) (void)hipHostFree(c->h_total);
+  if (c->h_flag) (void)hipHostFree(c->h_flag);
   for (int s = 0; s < kStages; s++) {
     if (c->h_stage[s]) (void)hipHostFree(c->h_stage[s]);
     if (c->d_stage[s]) (void)hipFree(c->d_stage[s]);
@@ -402,6 +406,8 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     m.counter = c->d_counter;
     m.total = r_total;
     m.host_total = c->h_total + ws;
+    m.host_flag = c->h_flag + ws;
+    m.seq = c->expect[ws] = ++c->seq;
     m.ln_inv_k = std::log(1.0 / (double)K);
     m.P = c->P;
     m.K = K;
@@ -428,6 +434,8 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     a.counter = c->d_counter;
     a.total = r_total;
     a.host_total = c->h_total + ws;
+    a.host_flag = c->h_flag + ws;
+    a.seq = c->expect[ws] = ++c->seq;
     a.pi0 = freqs[0];
     a.pi1 = freqs[1];
     a.pi2 = freqs[2];
@@ -492,6 +500,8 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     m.counter = c->d_counter;
     m.total = r_total;
     m.host_total = c->h_total + ws;
+    m.host_flag = c->h_flag + ws;
+    m.seq = c->expect[ws] = ++c->seq;
     m.ln_inv_k = std::log(1.0 / (double)K);
     m.P = c->P;
     m.K = K;
@@ -631,6 +641,8 @@ b3h_status run_propose_prop(b3h_calc *c, uint32_t nn, const uint32_t *nodes, uin
   m.counter = c->d_counter;
   m.total = c->d_total + ws;
   m.host_total = c->h_total + ws;
+  m.host_flag = c->h_flag + ws;
+  m.seq = c->expect[ws] = ++c->seq;
   m.ln_inv_k = std::log(1.0 / (double)K);
   m.P = c->P;
   m.K = K;
@@ -834,10 +846,14 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
                    ((size_t)K * 2 * c->I * host_mat + 255) / 256 * 256 +
                    ((size_t)K * 2 * c->I * 8 + 255) / 256 * 256 + ((size_t)K * 8 + 255) / 256 * 256 +
                    64 * 8 + 512;
-  if (hipHostMalloc((void **)&c->h_total, 64, hipHostMallocDefault) != hipSuccess) {
+  // fine-grained: the kernels publish the total before they retire
+  if (hipHostMalloc((void **)&c->h_total, 64, hipHostMallocCoherent) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_flag, 64, hipHostMallocCoherent) != hipSuccess) {
     fail(B3H_ERR_OOM, "hipHostMalloc failed");
     return bail(B3H_ERR_OOM);
   }
+  std::memset(c->h_total, 0, 64);
+  std::memset(c->h_flag, 0, 64);  // sequence number 0: nothing computed yet
   for (int s = 0; s < kStages; s++) {
     if (hipHostMalloc((void **)&c->h_stage[s], c->stage_bytes, hipHostMallocDefault) != hipSuccess) {
       fail(B3H_ERR_OOM, "hipHostMalloc staging failed");
@@ -1051,6 +1067,8 @@ b3h_status b3h_set_classes(b3h_calc *c, const uint32_t *classes) {
   m.counter = c->d_counter;
   m.total = c->d_total + slot;
   m.host_total = c->h_total + slot;
+  m.host_flag = c->h_flag + slot;
+  m.seq = c->expect[slot] = ++c->seq;
   m.ln_inv_k = std::log(1.0 / (double)c->K);
   m.P = c->P;
   m.K = c->K;
@@ -1136,10 +1154,28 @@ b3h_status b3h_total(b3h_calc *c, double *out) {
   uint32_t slot;
   if ((st = result_slot(c, &slot))) return st;
   if ((st = set_device(c))) return st;
-  // every kernel that writes a slot's total also writes it to the slot's
-  // pinned host mirror: wait for the stream, no copy
-  HIPCHK(hipStreamSynchronize(c->stream));
-  *out = *(volatile double *)(c->h_total + slot);
+  // every kernel that writes a slot's total also stores it in the slot's
+  // pinned host mirror and then the launch's sequence number: poll for the
+  // number (up to 20 ms), else wait for the stream, which also reports any
+  // error of the launch
+  const uint32_t want = c->expect[slot];
+  bool seen = __atomic_load_n(c->h_flag + slot, __ATOMIC_ACQUIRE) == want;
+  if (!seen) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 1; !seen; i++) {
+      __builtin_ia32_pause();
+      seen = __atomic_load_n(c->h_flag + slot, __ATOMIC_ACQUIRE) == want;
+      if (!seen && (i & 1023u) == 0 &&
+          std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20))
+        break;
+    }
+  }
+  if (!seen) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (__atomic_load_n(c->h_flag + slot, __ATOMIC_ACQUIRE) != want)
+      return fail(B3H_ERR_STATE, "no total was computed for the current state");
+  }
+  *out = *(volatile double *)(c->h_total + slot);  // ordered by the flag's acquire
   return B3H_OK;
 }
 

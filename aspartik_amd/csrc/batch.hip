@@ -406,6 +406,8 @@ b3h_status b3h_batch_propose_times(b3h_batch *b, const b3h_update *ups, uint32_t
     a.counter = b->counters + (size_t)i * 16;
     a.total = b->totals + i;
     a.host_total = nullptr;
+    a.host_flag = nullptr;
+    a.seq = 0;
     a.pik = nullptr;
     a.classes = nullptr;
     a.ln_inv_k = std::log(1.0 / (double)K);

@@ -52,6 +52,8 @@ struct F32Args {
   uint32_t *counter;      // last-tile ticket (self-resetting)
   double *total;
   double *host_total;     // pinned host mirror of *total, or null
+  uint32_t *host_flag;    // with host_total: seq stored after it (system-scope release)
+  uint32_t seq;
   double pi0, pi1, pi2, pi3;
   const double *pik;        // [K][4] root frequencies per category, or null (pi0..pi3)
   const uint8_t *classes;   // [P] category selected per pattern, or null (mean over categories)
@@ -162,6 +164,8 @@ struct MixArgs {
   uint32_t *counter;   // last-block ticket (self-resetting)
   double *total;
   double *host_total;     // pinned host mirror of *total, or null
+  uint32_t *host_flag;    // with host_total: seq stored after it (system-scope release)
+  uint32_t seq;
   double ln_inv_k;
   uint32_t P, K;
 };

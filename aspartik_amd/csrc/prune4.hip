@@ -441,7 +441,8 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
   double bsum = 0.0;
   if (tid == 0)
     for (uint32_t tt = 0; tt < T; tt++) bsum += red[tt];
-  last_block_total(bsum, a.blocksum, a.counter, a.total, a.host_total, flag, tile, a.ntiles);
+  last_block_total(bsum, a.blocksum, a.counter, a.total, a.host_total, a.host_flag, a.seq, flag,
+                   tile, a.ntiles);
 #undef B3H_FETCH
 #undef B3H_STORE
 #undef B3H_FETCH_HBM
@@ -705,7 +706,8 @@ __global__ __launch_bounds__(1024) void prune_path_kernel(F32Args arg, const uin
   double bsum = 0.0;
   if (tid == 0)
     for (uint32_t tt = 0; tt < T; tt++) bsum += red[tt];
-  last_block_total(bsum, a.blocksum, a.counter, a.total, a.host_total, flag, blockIdx.x, gridDim.x);
+  last_block_total(bsum, a.blocksum, a.counter, a.total, a.host_total, a.host_flag, a.seq, flag,
+                   blockIdx.x, gridDim.x);
 }
 
 uint32_t path_tiles(uint32_t P, uint32_t K) {

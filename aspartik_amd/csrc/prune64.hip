@@ -131,7 +131,8 @@ __global__ __launch_bounds__(256) void mix_kernel(MixArgs a) {
   __syncthreads();
   double b = 0.0;
   if (threadIdx.x == 0) b = ((red[0] + red[1]) + red[2]) + red[3];
-  last_block_total(b, a.blocksum, a.counter, a.total, a.host_total, &flag, blockIdx.x, gridDim.x);
+  last_block_total(b, a.blocksum, a.counter, a.total, a.host_total, a.host_flag, a.seq, &flag,
+                   blockIdx.x, gridDim.x);
 }
 
 uint32_t mix_blocks(uint32_t P) { return (P + 255) / 256; }

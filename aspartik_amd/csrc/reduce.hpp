@@ -28,7 +28,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 // the same order-only idiom.
 __device__ __forceinline__ void last_block_total(double block_value, double *blocksum,
                                                  uint32_t *counter, double *total,
-                                                 double *host_total, int *flag, uint32_t index,
+                                                 double *host_total, uint32_t *host_flag,
+                                                 uint32_t seq, int *flag, uint32_t index,
                                                  uint32_t count) {
   if (threadIdx.x == 0) {
     __hip_atomic_store(&blocksum[index], block_value, __ATOMIC_RELAXED,
@@ -49,9 +50,13 @@ __device__ __forceinline__ void last_block_total(double block_value, double *blo
     acc = wave_sum(acc);
     if (threadIdx.x == 0) {
       *total = acc;
-      // pinned host memory: visible to the host once the kernel has
-      // completed (after a stream synchronisation), with no copy
-      if (host_total) *host_total = acc;
+      // fine-grained pinned host memory: the value, then the sequence
+      // number with a system-scope release, so a host that sees the number
+      // also sees the value, before the kernel has retired
+      if (host_total) {
+        *host_total = acc;
+        __hip_atomic_store(host_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
       __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }

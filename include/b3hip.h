@@ -236,8 +236,10 @@ b3h_status b3h_site_mixed(b3h_calc *calc, double *out);
 
 /* The likelihood layer's total: sum_i w_i * mixed_i, reduced in FP64 on the
  * device (dna.rs:96-97, gamma.rs:156-166).  Blocks until the last propose
- * finishes; the reducing kernel has already stored the value in pinned host
- * memory, so no copy is issued. */
+ * has produced it: the reducing kernel stores the value and then a sequence
+ * number in pinned host memory, and this call polls for the number (after
+ * 20 ms it waits for the stream instead, which also reports launch errors).
+ * No copy is issued. */
 b3h_status b3h_total(b3h_calc *calc, double *out);
 
 /* Calculator::num_patterns (calculator/mod.rs:74-75). */
