@@ -11,6 +11,7 @@
 // minibatch is a dozen launches.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -253,7 +254,7 @@ __global__ __launch_bounds__(256) void k_predict1(FusedInput in, uint32_t D, uin
                                                   const float *b1, const uint16_t *W2t,
                                                   const float *b2, const uint16_t *w3,
                                                   const float *b3, float ymean, float ystd,
-                                                  float *out) {
+                                                  float *out, uint32_t *flag, uint32_t seq) {
   __shared__ float xs[kMaxFusedD];
   __shared__ float h1[kMaxFusedH];
   __shared__ float part[4];
@@ -271,7 +272,11 @@ __global__ __launch_bounds__(256) void k_predict1(FusedInput in, uint32_t D, uin
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
   if ((tid & 63u) == 0) part[tid >> 6] = s;
   __syncthreads();
-  if (tid == 0) out[0] = (part[0] + part[1] + part[2] + part[3] + b3[0]) * ystd + ymean;
+  if (tid == 0) {
+    out[0] = (part[0] + part[1] + part[2] + part[3] + b3[0]) * ystd + ymean;
+    // fine-grained pinned memory: a host that sees seq also sees the value
+    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 __global__ void k_cast(const float *p, uint16_t *pb, size_t n) {
@@ -309,6 +314,8 @@ struct b3m_mlp {
   float *xin = nullptr;
   uint32_t *idx = nullptr;
   float *h_out = nullptr;  // pinned result of the fused single-sample pass
+  uint32_t *h_flag = nullptr;  // sequence number stored after the result
+  uint32_t seq = 0;
   uint64_t adam_t = 0, seed = 0;
   std::vector<void *> owned;
 };
@@ -408,9 +415,26 @@ b3h_status mlp_predict(b3m_mlp *m, const float *x, uint32_t B, float *y) {
     std::memcpy(in.x, x, (size_t)m->D * 4);
     hipLaunchKernelGGL(k_predict1, dim3(1), dim3(256), 0, m->stream, in, m->D, m->Dp, m->H,
                        m->Hp, m->xmean, m->xstd, m->w1t, m->p + m->ob1, m->w2t, m->p + m->ob2,
-                       m->pb + m->ow3, m->p + m->ob3, m->ymean, m->ystd, m->h_out);
+                       m->pb + m->ow3, m->p + m->ob3, m->ymean, m->ystd, m->h_out, m->h_flag,
+                       ++m->seq);
     MCHK(hipGetLastError());
-    MCHK(hipStreamSynchronize(m->stream));
+    // poll for the launch's sequence number (up to 20 ms), else wait for
+    // the stream, which also reports any error of the launch
+    const uint32_t want = m->seq;
+    bool seen = false;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 1; !seen; i++) {
+      seen = __atomic_load_n(m->h_flag, __ATOMIC_ACQUIRE) == want;
+      if (!seen) __builtin_ia32_pause();
+      if (!seen && (i & 1023u) == 0 &&
+          std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20))
+        break;
+    }
+    if (!seen) {
+      MCHK(hipStreamSynchronize(m->stream));
+      if (__atomic_load_n(m->h_flag, __ATOMIC_ACQUIRE) != want)
+        return b3h::host_fail(B3H_ERR_HIP, "surrogate forward did not complete");
+    }
     y[0] = *(volatile float *)m->h_out;
     return B3H_OK;
   }
@@ -495,7 +519,12 @@ b3h_status b3m_mlp_create(int device, uint32_t inputs, uint32_t hidden, uint64_t
                        m->p, m->pb, m->np);
     ok = hipStreamSynchronize(m->stream) == hipSuccess;
   }
-  if (ok) ok = hipHostMalloc((void **)&m->h_out, 64, hipHostMallocDefault) == hipSuccess;
+  if (ok) ok = hipHostMalloc((void **)&m->h_out, 64, hipHostMallocCoherent) == hipSuccess &&
+               hipHostMalloc((void **)&m->h_flag, 64, hipHostMallocCoherent) == hipSuccess;
+  if (ok) {
+    std::memset(m->h_out, 0, 64);
+    std::memset(m->h_flag, 0, 64);
+  }
   if (!ok) return fail(b3h::host_fail(B3H_ERR_HIP, "surrogate initialisation failed"));
   *out = m;
   return B3H_OK;
@@ -507,6 +536,7 @@ b3h_status b3m_mlp_destroy(b3m_mlp *m) {
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   for (void *p : m->owned) (void)hipFree(p);
   if (m->h_out) (void)hipHostFree(m->h_out);
+  if (m->h_flag) (void)hipHostFree(m->h_flag);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return B3H_OK;
