@@ -153,6 +153,8 @@ SIGNATURES = {
     "b3h_device_count": (C.c_int, []),
     "b3h_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "b3h_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "b3h_profile_launch_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64,
+                                           C.POINTER(C.c_uint64)]),
     "b3h_device_bytes": (C.c_uint64, [C.c_void_p]),
     "b3h_gamma_rates": (C.c_int, [C.c_double, C.c_uint32, f64p]),
     "b3h_batch_create": (C.c_int, [C.POINTER(Config), C.c_uint32, C.POINTER(Instance), C.c_uint32,

@@ -215,6 +215,15 @@ class HipCalculator:
         check(lib().b3h_profile_read(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def profile_launch_times(self):
+        """Per-launch kernel times (ms) since profile_enable, in launch order."""
+        n = C.c_uint64()
+        check(lib().b3h_profile_launch_times(self._h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.float64)
+        if n.value:
+            check(lib().b3h_profile_launch_times(self._h, out.ctypes.data, n.value, C.byref(n)))
+        return out
+
     def device_bytes(self) -> int:
         return int(lib().b3h_device_bytes(self._h))
 

@@ -113,6 +113,7 @@ struct b3h_calc {
   size_t ev_used = 0;
   double prof_ms = 0.0;
   uint64_t prof_launches = 0;
+  std::vector<double> prof_each;  // per-launch ms since enable
 };
 
 namespace {
@@ -222,6 +223,16 @@ unsigned char *next_stage(b3h_calc *c, int *which) {
   return c->h_stage[s];
 }
 
+// Restores the EditBuf selectors flipped by a propose unless disarmed (the
+// pre-propose copy is kept in editbuf_next after the swap).
+struct EditbufUndo {
+  b3h_calc *c;
+  bool armed = true;
+  ~EditbufUndo() {
+    if (armed) c->editbuf.swap(c->editbuf_next);
+  }
+};
+
 b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
                        const uint32_t *children, const double *mats,
                        const double *times, const double *rates,
@@ -280,6 +291,9 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
       !c->sched.build(n, internals, children, c->L, c->editbuf_next.data(), reorder, cap, &err))
     return fail(B3H_ERR_ARG, err);
   c->editbuf.swap(c->editbuf_next);
+  // every error return below restores the slot selectors, so a failed
+  // propose leaves the accepted state readable (no proposal is pending)
+  EditbufUndo undo{c};
 
   // stage: entries | matrices (mode 0) or times + rates (mode 1) or branch
   // lengths [K][2n] (mode 2) | pi (the MFMA path stages host matrices
@@ -508,6 +522,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     HIPCHK(launch_mix(m, c->stream));
     if (e1) HIPCHK(hipEventRecord(e1, c->stream));
   }
+  undo.armed = false;
   c->has_result = true;
   c->pending = true;
   return B3H_OK;
@@ -541,6 +556,7 @@ b3h_status run_propose_prop(b3h_calc *c, uint32_t nn, const uint32_t *nodes, uin
     m = (uint8_t)(((m & 1u) ^ (uint8_t)~(m >> 1)) & 3u);
   }
   c->editbuf.swap(c->editbuf_next);
+  EditbufUndo undo{c};
   auto slot = [&](uint32_t node) { return node * 2u + (c->editbuf[node] & 1u); };
 
   const uint32_t nm = nn - 1;
@@ -647,6 +663,7 @@ b3h_status run_propose_prop(b3h_calc *c, uint32_t nn, const uint32_t *nodes, uin
   m.P = c->P;
   m.K = K;
   HIPCHK(launch_mix(m, c->stream));
+  undo.armed = false;
   c->has_result = true;
   c->pending = true;
   return B3H_OK;
@@ -1192,23 +1209,46 @@ b3h_status b3h_profile_enable(b3h_calc *c, int enable) {
   c->ev_used = 0;
   c->prof_ms = 0.0;
   c->prof_launches = 0;
+  c->prof_each.clear();
   return B3H_OK;
 }
 
-b3h_status b3h_profile_read(b3h_calc *c, double *kernel_ms, uint64_t *launches) {
-  b3h_status st = check_handle(c);
-  if (st) return st;
-  if ((st = set_device(c))) return st;
+namespace {
+// Fold the recorded event pairs into the running sums and the per-launch list.
+b3h_status profile_collect(b3h_calc *c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < c->ev_used; i++) {
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev_start[i], c->ev_stop[i]));
     c->prof_ms += ms;
     c->prof_launches++;
+    c->prof_each.push_back(ms);
   }
   c->ev_used = 0;
+  return B3H_OK;
+}
+}  // namespace
+
+b3h_status b3h_profile_read(b3h_calc *c, double *kernel_ms, uint64_t *launches) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if ((st = set_device(c))) return st;
+  if ((st = profile_collect(c))) return st;
   if (kernel_ms) *kernel_ms = c->prof_ms;
   if (launches) *launches = c->prof_launches;
+  return B3H_OK;
+}
+
+b3h_status b3h_profile_launch_times(b3h_calc *c, double *ms, uint64_t capacity,
+                                    uint64_t *launches) {
+  b3h_status st = check_handle(c);
+  if (st) return st;
+  if (capacity && !ms) return fail(B3H_ERR_ARG, "null output with nonzero capacity");
+  if ((st = set_device(c))) return st;
+  if ((st = profile_collect(c))) return st;
+  const uint64_t n = c->prof_each.size();
+  for (uint64_t i = 0; i < n && i < capacity; i++) ms[i] = c->prof_each[i];
+  if (launches) *launches = n;
   return B3H_OK;
 }
 

@@ -801,14 +801,20 @@ b3h_status b3m_chain_run(b3m_chain *c, uint64_t steps, uint32_t mode, uint32_t r
         s.prior_rejects++;
       } else if (mode == B3M_DELAYED && (op != kNarrow || c->cfg.screen_topology)) {
         double fnew = 0.0;
-        if ((st = surrogate(&fnew))) break;
+        if ((st = surrogate(&fnew))) {
+          finalize(c, false);  // an error ends the run with the proposal rejected
+          break;
+        }
         const double df = c->cfg.surrogate_beta * (fnew - c->surrogate_cur);
         const double r1 = df + (prior - c->logprior) + hastings;
         if (!(r1 > std::log(c->rng.uniform()))) {
           s.stage1_rejects++;
         } else {
           double ll = 0.0;
-          if ((st = exact(&ll))) break;
+          if ((st = exact(&ll))) {
+            finalize(c, false);
+            break;
+          }
           if (ll != -INFINITY && (ll - c->loglik) - df > std::log(c->rng.uniform())) {
             accept = true;
             c->loglik = ll;
@@ -821,7 +827,10 @@ b3h_status b3m_chain_run(b3m_chain *c, uint64_t steps, uint32_t mode, uint32_t r
         }
       } else {
         double ll = 0.0;
-        if ((st = exact(&ll))) break;
+        if ((st = exact(&ll))) {
+          finalize(c, false);
+          break;
+        }
         if (ll != -INFINITY) {
           const double ratio = ll + prior - c->posterior + hastings;
           if (ratio > std::log(c->rng.uniform())) {  // mcmc.rs:346-359
@@ -834,7 +843,11 @@ b3h_status b3m_chain_run(b3m_chain *c, uint64_t steps, uint32_t mode, uint32_t r
       }
     }
     if (accept && mode == B3M_DELAYED && op == kNarrow && !c->cfg.screen_topology) {
-      if ((st = surrogate(&c->surrogate_cur))) break;
+      if ((st = surrogate(&c->surrogate_cur))) {
+        finalize(c, true);  // the move was accepted; only its surrogate value failed
+        c->surrogate_valid = false;
+        break;
+      }
     }
     finalize(c, accept);
     if (accept) {

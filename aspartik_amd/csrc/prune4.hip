@@ -244,14 +244,14 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
       const size_t o_ = (size_t)(ls_ & SRC_IDX) * KP + (size_t)c * a.P;        \
       _Pragma("unroll") for (int q = 0; q < PPL; q++) {                        \
         hl[q] = a.partials[o_ + pp[q]];                                        \
-        hel[q] = a.exps[o_ + pp[q]];                                           \
+        hel[q] = exp_unpack(a.exps[o_ + pp[q]]);                               \
       }                                                                        \
     }                                                                          \
     if ((rs_ & SRC_TAG) == SRC_HBM) {                                          \
       const size_t o_ = (size_t)(rs_ & SRC_IDX) * KP + (size_t)c * a.P;        \
       _Pragma("unroll") for (int q = 0; q < PPL; q++) {                        \
         hr[q] = a.partials[o_ + pp[q]];                                        \
-        her[q] = a.exps[o_ + pp[q]];                                           \
+        her[q] = exp_unpack(a.exps[o_ + pp[q]]);                               \
       }                                                                        \
     }                                                                          \
   } while (0)
@@ -305,14 +305,14 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
             le[q] = E[q];
           } else if (tag == SRC_LDS) {
             lv[q] = apply(ML, stk[(idx * PPL + q) * 64 + lane]);
-            le[q] = stke[(idx * PPL + q) * 64 + lane];
+            le[q] = exp_unpack(stke[(idx * PPL + q) * 64 + lane]);
           } else if (HBM) {
             lv[q] = apply(ML, cl[q]);
             le[q] = cel[q];
           } else {
             const size_t o = (size_t)idx * KP + (size_t)c * a.P + pp[q];
             lv[q] = apply(ML, a.partials[o]);
-            le[q] = a.exps[o];
+            le[q] = exp_unpack(a.exps[o]);
           }
         }
       }
@@ -328,14 +328,14 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
             re[q] = E[q];
           } else if (tag == SRC_LDS) {
             rv[q] = apply(MR, stk[(idx * PPL + q) * 64 + lane]);
-            re[q] = stke[(idx * PPL + q) * 64 + lane];
+            re[q] = exp_unpack(stke[(idx * PPL + q) * 64 + lane]);
           } else if (HBM) {
             rv[q] = apply(MR, cr[q]);
             re[q] = cer[q];
           } else {
             const size_t o = (size_t)idx * KP + (size_t)c * a.P + pp[q];
             rv[q] = apply(MR, a.partials[o]);
-            re[q] = a.exps[o];
+            re[q] = exp_unpack(a.exps[o]);
           }
         }
       }
@@ -354,21 +354,21 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
         v.z = __builtin_amdgcn_ldexpf(v.z, -ex);
         v.w = __builtin_amdgcn_ldexpf(v.w, -ex);
         x[q] = v;
-        E[q] = le[q] + re[q] + ex;
+        E[q] = exp_clamp(le[q] + re[q] + ex);
         if (active[q]) {
           const size_t o = (size_t)out * KP + (size_t)c * a.P + p[q];
 #if B3H_NT_STORE
           const f4v xv = {v.x, v.y, v.z, v.w};
           __builtin_nontemporal_store(xv, reinterpret_cast<f4v *>(&a.partials[o]));
-          __builtin_nontemporal_store((int16_t)E[q], &a.exps[o]);
+          __builtin_nontemporal_store(exp_pack(E[q]), &a.exps[o]);
 #else
           a.partials[o] = v;
-          a.exps[o] = (int16_t)E[q];
+          a.exps[o] = exp_pack(E[q]);
 #endif
         }
         if (push != NO_PUSH) {
           stk[(push * PPL + q) * 64 + lane] = v;
-          stke[(push * PPL + q) * 64 + lane] = (int16_t)E[q];
+          stke[(push * PPL + q) * 64 + lane] = exp_pack(E[q]);
         }
       }
     }
@@ -395,7 +395,7 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
       v = fma((double)x[q].y, pi1, v);
       v = fma((double)x[q].z, pi2, v);
       v = fma((double)x[q].w, pi3, v);
-      site = fma((double)E[q], LN2, log(v));
+      site = exp_site(v, E[q]);
       a.site[(size_t)c * a.P + p[q]] = site;
     }
     mix[(w * PPL + q) * 64 + lane] = site;
@@ -575,7 +575,7 @@ __global__ __launch_bounds__(1024) void prune_path_kernel(F32Args arg, const uin
     if ((sv & SRC_TAG) == SRC_HBM) {
       const size_t o = (size_t)idx * KP + (size_t)c * a.P + pp;
       v = a.partials[o];
-      e = a.exps[o];
+      e = exp_unpack(a.exps[o]);
     } else {
       v = make_float4(0.f, 0.f, 0.f, 0.f);
       e = codes8[(size_t)idx * a.cstride + pp];
@@ -640,16 +640,16 @@ __global__ __launch_bounds__(1024) void prune_path_kernel(F32Args arg, const uin
         v.z = __builtin_amdgcn_ldexpf(v.z, -ex);
         v.w = __builtin_amdgcn_ldexpf(v.w, -ex);
         x = v;
-        E = ce + se + ex;
+        E = exp_clamp(ce + se + ex);
         if (active) {
           const size_t o = (size_t)e4.x * KP + (size_t)c * a.P + p;
 #if B3H_NT_STORE
           const f4v xv = {v.x, v.y, v.z, v.w};
           __builtin_nontemporal_store(xv, reinterpret_cast<f4v *>(&a.partials[o]));
-          __builtin_nontemporal_store((int16_t)E, &a.exps[o]);
+          __builtin_nontemporal_store(exp_pack(E), &a.exps[o]);
 #else
           a.partials[o] = v;
-          a.exps[o] = (int16_t)E;
+          a.exps[o] = exp_pack(E);
 #endif
         }
       }
@@ -670,7 +670,7 @@ __global__ __launch_bounds__(1024) void prune_path_kernel(F32Args arg, const uin
     v = fma((double)x.y, pi1, v);
     v = fma((double)x.z, pi2, v);
     v = fma((double)x.w, pi3, v);
-    site = fma((double)E, LN2, log(v));
+    site = exp_site(v, E);
     a.site[(size_t)c * a.P + p] = site;
   }
   // LDS: mix [T][K][64], then the sub-tile sums [T] and the ticket flag

@@ -223,7 +223,7 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
       for (uint32_t s = 0; s < 4; s++)
         y[side].v[s] = *reinterpret_cast<const h8 *>(&tbuf[t][side][s * 512u + lane * 8u]);
       code[side] = reinterpret_cast<const uint8_t *>(&cbuf[t][side][0])[col];
-      ey[side] = reinterpret_cast<const int16_t *>(&ebuf[t][side][0])[col];
+      ey[side] = exp_unpack(reinterpret_cast<const int16_t *>(&ebuf[t][side][0])[col]);
     }
   };
 
@@ -330,7 +330,7 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
     const int ex = mx > 0.0f ? __builtin_amdgcn_frexp_expf(mx) : 0;
 #pragma unroll
     for (uint32_t r = 0; r < 32; r++) v[r] = __builtin_amdgcn_ldexpf(v[r], kXTop - ex);
-    E = ce[0] + ce[1] + ex - kXTop;
+    E = exp_clamp(ce[0] + ce[1] + ex - kXTop);
 
     if (more) {
       Tile x;
@@ -351,7 +351,7 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
 #pragma unroll
       for (uint32_t s = 0; s < 4; s++)
         __builtin_nontemporal_store(__builtin_bit_cast(u4, x.v[s]), g + s * 64u);
-      __builtin_nontemporal_store((int16_t)E, exp_ptr(out));
+      __builtin_nontemporal_store(exp_pack(E), exp_ptr(out));
       lds_barrier();  // the next entry's matrices are in LDS
     } else {
       // root of the update: ln(sum_j pi_j x_j) + E ln 2 in FP64 from the
@@ -361,7 +361,7 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
 #pragma unroll
       for (uint32_t r = 0; r < 32; r++) sum = fma((double)v[r], pl[row_of(0, r)], sum);
       sum = pair_sum(sum);
-      if (p < a.P && h == 0) a.site[(size_t)c * a.P + p] = fma((double)E, LN2, log(sum));
+      if (p < a.P && h == 0) a.site[(size_t)c * a.P + p] = exp_site(sum, E);
     }
   }
 }

@@ -111,21 +111,21 @@ __global__ __launch_bounds__(256) void prune_prop_kernel(PropArgs a) {
       if (old == 0 && scale) ss += a.scale_ln;
       else if (old == 1 && !scale) ss -= a.scale_ln;
     } else {
-      const int el = (e.left == prev) ? pe : a.exps[(size_t)e.left * KP + cp];
-      const int er = (e.right == prev) ? pe : a.exps[(size_t)e.right * KP + cp];
+      const int el = (e.left == prev) ? pe : exp_unpack(a.exps[(size_t)e.left * KP + cp]);
+      const int er = (e.right == prev) ? pe : exp_unpack(a.exps[(size_t)e.right * KP + cp]);
       const float mx = fmaxf(fmaxf((float)x.x, (float)x.y), fmaxf((float)x.z, (float)x.w));
       const int ex = (mx > 0.0f) ? __builtin_amdgcn_frexp_expf(mx) : 0;
       x.x = (T)__builtin_amdgcn_ldexpf((float)x.x, -ex);
       x.y = (T)__builtin_amdgcn_ldexpf((float)x.y, -ex);
       x.z = (T)__builtin_amdgcn_ldexpf((float)x.z, -ex);
       x.w = (T)__builtin_amdgcn_ldexpf((float)x.w, -ex);
-      E = el + er + ex;
+      E = exp_clamp(el + er + ex);
     }
     const T *M = mats + ((size_t)i * a.K + c) * 16;
     const Vec4<T> pr = {row_dot<T>(M, x), row_dot<T>(M + 4, x), row_dot<T>(M + 8, x),
                         row_dot<T>(M + 12, x)};
     store4<T>(a.proj, (size_t)e.out * KP + cp, pr);
-    if (!F64) a.exps[(size_t)e.out * KP + cp] = (int16_t)E;
+    if (!F64) a.exps[(size_t)e.out * KP + cp] = exp_pack(E);
     prev = e.out;
     pv = pr;
     pe = E;
@@ -151,13 +151,13 @@ __global__ __launch_bounds__(256) void prune_prop_kernel(PropArgs a) {
     site = log(lk) - (double)ss;
     a.scale_sums[cp] = ss;
   } else {
-    const int el = (e.left == prev) ? pe : a.exps[(size_t)e.left * KP + cp];
-    const int er = (e.right == prev) ? pe : a.exps[(size_t)e.right * KP + cp];
+    const int el = (e.left == prev) ? pe : exp_unpack(a.exps[(size_t)e.left * KP + cp]);
+    const int er = (e.right == prev) ? pe : exp_unpack(a.exps[(size_t)e.right * KP + cp]);
     double lk = ((double)l.x * (double)r.x) * pi0;
     lk = fma((double)l.y * (double)r.y, pi1, lk);
     lk = fma((double)l.z * (double)r.z, pi2, lk);
     lk = fma((double)l.w * (double)r.w, pi3, lk);
-    site = fma((double)(el + er), LN2, log(lk));
+    site = exp_site(lk, exp_clamp(el + er));
   }
   a.site[cp] = site;
 }

@@ -92,6 +92,52 @@ def algorithmic_bytes(internals, children, num_leaves, root, K, P, states=4, ele
     return B + 4 * P + 8
 
 
+def store_bytes(internals, children, num_leaves, K, P, cherries_stored=True, unit=18,
+                reread_in_pass=False, epilogue=True):
+    """HBM bytes one evaluation of this update list must move with this
+    library's storage (the roofline basis of the pruning kernels):
+      - every listed node's partial written once: ``unit`` bytes per pattern
+        and category (FP32: a float4 and an int16 exponent, 18 B; codon FP16:
+        64 padded states and an exponent, 130 B).  A cherry, both of whose
+        children are leaves, is written only when ``cherries_stored``;
+      - every stored child that the pass does not compute read once (``unit``
+        bytes per pattern and category; a leaf: its code byte per pattern);
+      - with ``reread_in_pass`` (the codon kernel, which has no LDS stack), the
+        first-evaluated internal child of a node whose children are both
+        internal is read back as well;
+      - with ``epilogue``, the pattern weights read (4 B) and the site values
+        written (8 B per category plus 8 B mixed) per pattern.
+    A child computed in the same pass is otherwise consumed from registers or
+    LDS and not read back."""
+    internals = np.asarray(internals).reshape(-1)
+    children = np.asarray(children).reshape(-1, 2)
+    listed = set(int(n) for n in internals)
+    part = unit * K * P
+    B = 0
+    leaves_read = set()
+    for n, ch in zip(internals, children):
+        cherry = ch[0] < num_leaves and ch[1] < num_leaves
+        if cherries_stored or not cherry:
+            B += part
+        both_internal = True
+        for c in ch:
+            c = int(c)
+            if c < num_leaves:
+                leaves_read.add(c)
+                both_internal = False
+            elif c not in listed:
+                B += part
+                both_internal = False
+        if reread_in_pass and both_internal:
+            B += part
+    B += len(leaves_read) * P
+    if epilogue:
+        B += 4 * P + 8 * K * P + 8 * P
+    else:
+        B += 8 * K * P
+    return B
+
+
 def algorithmic_flops(internals, children, num_leaves, K, P, states=4):
     """Per (node, pattern, category): sum over internal children of 2 S^2,
     plus S for the product; at the root 2S + 1 more."""

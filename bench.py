@@ -57,10 +57,20 @@ def read_traffic(workload, kernel="prune_f32"):
     return None
 
 
-def time_evals(calc, lists, times, rates, steps, warmup, sync_each=True):
+def time_evals(calc, lists, times, rates, steps, warmup, sync_each=True, prewarm_s=0.0):
+    """Time ``steps`` evaluations after ``prewarm_s`` seconds of back-to-back
+    evaluations (so the device is at steady clocks whatever ``warmup`` is)
+    and ``warmup`` more.  Returns (wall seconds, last total, mean kernel ms,
+    per-launch kernel ms)."""
     internals, children = lists
     # every evaluation is followed by accept(), as in an MCMC step: b3's FP64
     # scale counts are not idempotent across proposes without one (cpu4.rs:78-89)
+    t_end = time.perf_counter() + prewarm_s
+    while time.perf_counter() < t_end:
+        for _ in range(10):
+            calc.propose_times(internals, children, times, rates)
+            calc.total()
+            calc.accept()
     for _ in range(warmup):
         calc.propose_times(internals, children, times, rates)
         calc.total()
@@ -74,9 +84,21 @@ def time_evals(calc, lists, times, rates, steps, warmup, sync_each=True):
         calc.accept()
     total = calc.total()
     dt = time.perf_counter() - t0
-    kms, launches = calc.profile_read()
+    each = calc.profile_launch_times()
     calc.profile_enable(False)
-    return dt, total, kms / max(1, launches)
+    return dt, total, float(each.mean()) if each.size else 0.0, each
+
+
+def cpu_threads(K, P):
+    """Threads of the CPU baseline: b3 runs K category calculators at once
+    (gamma.rs:69,143-152), each with min(cores, ceil(P / 1000)) threads
+    (cpu4.rs:127-132).  On a host with fewer cores than that product the
+    threads per category are cut so the total does not exceed the cores:
+    oversubscribing the cores does not add throughput."""
+    ncores = len(os.sched_getaffinity(0))
+    tpc = min(ncores, (P + 999) // 1000)
+    tpc = max(1, min(tpc, ncores // K))
+    return ncores, tpc
 
 
 def bench_c1(args):
@@ -164,7 +186,7 @@ def bench_batch(args):
         t = DatedTree(x["L"], x["parents"], x["children"], x["heights"])
         internals, children = workloads.full_lists(t)
         ups.append((internals, children, workloads.edge_times(t, children), rates))
-        B += workloads.algorithmic_bytes(internals, children, x["L"], t.root, K, x["P"])
+        B += workloads.store_bytes(internals, children, x["L"], K, x["P"])
     data = [(x["codes"], None) for x in insts]
     models = [(x["model"].q_matrix(), x["model"].frequencies(), True) for x in insts]
     log("[bench] batched: tuning tiles")
@@ -201,6 +223,7 @@ def bench_batch(args):
         order = np.random.default_rng(1).permutation(len(insts))
         ncores = len(os.sched_getaffinity(0))
         rel, nev, t_cpu, work, sample = [], 0, 0.0, 0, []
+        threads_used = 1
         for i in order:
             if t_cpu > args.cpu_seconds or len(sample) >= 24:
                 break
@@ -209,7 +232,8 @@ def bench_batch(args):
             x = insts[i]
             ot = O.Tree(x["L"], x["parents"], x["children"], x["heights"])
             om = O.Model.hky(x["kappa"], np.full(4, 0.25))
-            tpc = min(ncores, (x["P"] + 999) // 1000)
+            _, tpc = cpu_threads(K, x["P"])
+            threads_used = max(threads_used, tpc * K)
             lik = O.GammaLikelihood(ot, om, x["codes"], x["P"], np.ones(x["P"], np.uint32),
                                     num_categories=K, alpha=wl["alpha"],
                                     threads_per_category=tpc, parallel_categories=True)
@@ -227,7 +251,7 @@ def bench_batch(args):
         all_work = sum((x["L"] - 1) * x["P"] for x in insts)
         # instance evaluations per second at the average instance size
         cpu = {"value": (work / t_cpu) / (all_work / len(insts)), "unit": "instance logL evals/s",
-               "cores": ncores * K, "kind": "port",
+               "cores": threads_used, "host_cores": ncores, "kind": "port",
                "sample": f"{nev} full evaluations of {len(sample)} randomly chosen instances "
                          f"({t_cpu:.1f} s), scaled to the batch's mean instance size by "
                          f"node x pattern count"}
@@ -270,14 +294,18 @@ def bench_codon(args):
     pi = model.frequencies()
     calc = Calculator.HIPCodon(61).make(wl["codes"], P, K)
     calc.set_model(model.q_matrix(), pi)
-    dt, total, kernel_ms = time_evals(calc, full, times_full, rates, args.steps, args.warmup)
-    log(f"[bench] codon full: {args.steps / dt:.1f} evals/s, kernel {kernel_ms * 1e3:.1f} us")
+    dt, total, kernel_mean_ms, each = time_evals(calc, full, times_full, rates, args.steps,
+                                                 args.warmup, prewarm_s=args.prewarm)
+    kernel_ms = float(np.median(each))
+    log(f"[bench] codon full: {args.steps / dt:.1f} evals/s, kernel median {kernel_ms * 1e3:.1f} us")
     site_gpu = np.array([calc.likelihoods(k) for k in range(K)])
     mixed_gpu = calc.site_mixed()
     dev_bytes = calc.device_bytes()
     calc.close()
     root = tree.root
-    B = workloads.algorithmic_bytes(full[0], full[1], L, root, K, P, states=64, elem=2)
+    B = workloads.store_bytes(full[0], full[1], L, K, P, unit=130, reread_in_pass=True,
+                              epilogue=False)
+    B_ref = workloads.algorithmic_bytes(full[0], full[1], L, root, K, P, states=64, elem=2)
     F = workloads.algorithmic_flops(full[0], full[1], L, K, P, states=61)
     achieved = B / (kernel_ms * 1e-3) / 1e9
     cpu, err = None, None
@@ -333,7 +361,8 @@ def bench_codon(args):
                      "traffic": read_traffic(wl["name"], "prune_codon")},
         "mfma": {"useful_tflops": tflops, "issued_tflops": tflops * 2 * (64 / 61) ** 2,
                  "peak_fp16_tflops": 2500.0},
-        "cpu_baseline": cpu, "kernel_us": kernel_ms * 1e3, "algorithmic_bytes_per_eval": B,
+        "cpu_baseline": cpu, "kernel_us": kernel_ms * 1e3, "kernel_us_mean": kernel_mean_ms * 1e3,
+        "algorithmic_bytes_per_eval": B, "reference_algorithm_bytes_per_eval": B_ref,
         "algorithmic_flops_per_eval": F, "error_vs_fp64_oracle": err,
         "device_bytes": dev_bytes,
     }), flush=True)
@@ -455,7 +484,7 @@ def bench_mcmc(args):
         ncores = len(os.sched_getaffinity(0))
         otree = O.Tree(L, wl["parents"], wl["children"], wl["heights"])
         om = O.Model.gtr(wl["gtr_rates"], wl["freqs"])
-        tpc = min(ncores, (P + 999) // 1000)
+        _, tpc = cpu_threads(4, P)
         lik = O.GammaLikelihood(otree, om, wl["codes"], P, wl["weights"], num_categories=4,
                                 alpha=wl["alpha"], threads_per_category=tpc,
                                 parallel_categories=True)
@@ -473,7 +502,7 @@ def bench_mcmc(args):
         ps = res["plain"]["stats"]
         evals_per_ess = ps["exact_evals"] / res["plain"]["min_ess"]
         cpu = {"value": (nev / dtc) / evals_per_ess, "unit": "ESS/s (plain MH, upper bound)",
-               "cores": tpc * 4, "kind": "port",
+               "cores": tpc * 4, "host_cores": ncores, "kind": "port",
                "sample": f"{nev} full evaluations by the FP64 oracle ({dtc:.1f} s, "
                          f"{nev / dtc:.2f} evals/s), divided by the plain GPU chain's "
                          f"{evals_per_ess:.1f} exact evaluations per effective sample "
@@ -569,7 +598,7 @@ def bench_hetero(args):
     total = calc.total()
     calc.close()
 
-    B = workloads.algorithmic_bytes(full[0], full[1], L, tree.root, K, P)
+    B = workloads.store_bytes(full[0], full[1], L, K, P)
     achieved = B / (kernel_ms * 1e-3) / 1e9
     cpu = err = None
     if not args.no_cpu:
@@ -750,6 +779,9 @@ def main():
     ap.add_argument("--taxa", type=int, default=128)
     ap.add_argument("--patterns", type=int, default=100_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--prewarm", type=float, default=2.0,
+                    help="seconds of untimed evaluations before the warmup steps, so the "
+                         "timed steps run at steady clocks")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-f32", action="store_true", help="profiling runs: FP32 kernel only")
     ap.add_argument("--workload", choices=["gtr128", "c1", "batch", "codon", "mcmc", "hetero",
@@ -813,12 +845,15 @@ def main():
     calc = Calculator.HIP().make(wl["codes"], P, K)
     calc.set_weights(wl["weights"])
     calc.set_model(model.q_matrix(), model.frequencies(), True)
-    dt, total32, kernel_ms = time_evals(calc, full, times_full, rates, args.steps, args.warmup)
+    dt, total32, kernel_mean_ms, each = time_evals(calc, full, times_full, rates, args.steps,
+                                                   args.warmup, prewarm_s=args.prewarm)
+    kernel_ms = float(np.median(each))
     evals_per_s = args.steps / dt
-    log(f"[bench] fp32 full: {evals_per_s:.1f} evals/s, kernel {kernel_ms * 1e3:.1f} us")
+    log(f"[bench] fp32 full: {evals_per_s:.1f} evals/s, kernel median {kernel_ms * 1e3:.1f} us "
+        f"(mean {kernel_mean_ms * 1e3:.1f} us) after {args.prewarm:.1f} s pre-warm")
 
     # pipelined (no host round trip per evaluation)
-    dt_pipe, _, _ = time_evals(calc, full, times_full, rates, args.steps, 2, sync_each=False)
+    dt_pipe, _, _, _ = time_evals(calc, full, times_full, rates, args.steps, 2, sync_each=False)
 
     # partial evaluations: one random non-root edge and its path to the root
     rng = np.random.default_rng(99)
@@ -857,13 +892,18 @@ def main():
         c64.set_weights(wl["weights"])
         c64.set_model(model.q_matrix(), model.frequencies(), True)
         steps64 = max(20, args.steps // 4)
-        dt64, total64, k64 = time_evals(c64, full, times_full, rates, steps64, 5)
-        f64 = dict(evals_per_s=steps64 / dt64, kernel_us=k64 * 1e3, total=total64)
+        dt64, total64, _, e64 = time_evals(c64, full, times_full, rates, steps64, 5,
+                                           prewarm_s=min(1.0, args.prewarm))
+        f64 = dict(evals_per_s=steps64 / dt64, kernel_us=float(np.median(e64)) * 1e3,
+                   total=total64)
         c64.close()
         log(f"[bench] fp64 plain: {f64['evals_per_s']:.1f} evals/s")
 
-    # algorithmic bytes / flops of one full evaluation
-    B = workloads.algorithmic_bytes(full[0], full[1], L, root, K, P)
+    # bytes the evaluation must move with this storage (the roofline basis),
+    # the reference algorithm's byte count (every child read from memory) and
+    # flops of one full evaluation
+    B = workloads.store_bytes(full[0], full[1], L, K, P)
+    B_ref = workloads.algorithmic_bytes(full[0], full[1], L, root, K, P)
     F = workloads.algorithmic_flops(full[0], full[1], L, K, P)
     achieved = B / (kernel_ms * 1e-3) / 1e9
     traffic = read_traffic(workload_name)
@@ -875,8 +915,7 @@ def main():
         log("[bench] FP64 CPU oracle (cpu_baseline)")
         otree = O.Tree(L, wl["parents"], wl["children"], wl["heights"])
         om = O.Model.gtr(model.rates, model.freqs)
-        ncores = len(os.sched_getaffinity(0))
-        tpc = min(ncores, (P + 999) // 1000)  # cpu4.rs:127-132
+        ncores, tpc = cpu_threads(K, P)
         t0 = time.perf_counter()
         olik = O.GammaLikelihood(otree, om, wl["codes"], P, wl["weights"], num_categories=K,
                                  alpha=wl["alpha"], threads_per_category=tpc,
@@ -913,12 +952,13 @@ def main():
                 break
         dtp = time.perf_counter() - t0
         cpu = {"value": nev / dtc, "unit": "logL evals/s", "cores": tpc * K,
+               "host_cores": ncores, "threads": tpc * K,
                "kind": "port", "partial_evals_per_s": npc / dtp,
                "sample": f"{nev} full GTR+Gamma4 evaluations of the same {L}-taxon "
                          f"{P}-pattern workload ({dtc:.1f} s) and {npc} one-edge partial "
-                         f"evaluations ({dtp:.1f} s); {K} categories x {tpc} "
-                         f"threads as cpu4.rs/gamma.rs; host has {ncores} cores; "
-                         f"construction {t_first:.1f} s"}
+                         f"evaluations ({dtp:.1f} s); {K} categories run at once as "
+                         f"gamma.rs does, {tpc} threads each ({tpc * K} threads on "
+                         f"{ncores} host cores); construction {t_first:.1f} s"}
         rel_total = abs(total32 - ref_total) / abs(ref_total)
         per_site = np.abs(site32 - ref_sites) / np.maximum(1.0, np.abs(ref_sites))
         err = dict(total_rel=rel_total, max_site=float(per_site.max()), ref_total=ref_total,
@@ -947,7 +987,13 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
         "kernel_us": kernel_ms * 1e3,
+        "kernel_us_mean": kernel_mean_ms * 1e3,
+        "kernel_launches_timed": int(each.size),
+        "prewarm_s": args.prewarm,
         "algorithmic_bytes_per_eval": B,
+        "traffic_GBps": (traffic / (kernel_ms * 1e-3) / 1e9) if traffic else None,
+        "reference_algorithm_bytes_per_eval": B_ref,
+        "reference_algorithm_GBps": B_ref / (kernel_ms * 1e-3) / 1e9,
         "algorithmic_flops_per_eval": F,
         "pipelined_evals_per_s": args.steps / dt_pipe,
         "partial_evals_per_s": nparts / dt_part,

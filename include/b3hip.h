@@ -257,6 +257,10 @@ int b3h_device_count(void);
 b3h_status b3h_profile_enable(b3h_calc *calc, int enable);
 b3h_status b3h_profile_read(b3h_calc *calc, double *kernel_ms,
                             uint64_t *launches);
+/* The same launches one by one: ms[i] is the i-th timed launch since enable
+ * (up to `capacity` values); *launches is the count.  For medians. */
+b3h_status b3h_profile_launch_times(b3h_calc *calc, double *ms,
+                                    uint64_t capacity, uint64_t *launches);
 
 /* Discrete-gamma category rates (likelihood/gamma.rs:205-225): the
  * Gamma(alpha, 1) quantiles at (2i+1)/(2K), i = 0..K-1, divided by their

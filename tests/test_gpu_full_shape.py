@@ -1,0 +1,130 @@
+"""Parity at the full sizes of the measured workloads, against the FP64 oracle.
+
+- GTR+Gamma4, 128 taxa, 100,000 unique simulated patterns with weights
+  U{1..5}, seed 4 (the benchmark workload): a full evaluation and one-edge
+  partial evaluations after node-height moves, per site and in total.
+- Batched HKY+Gamma4: 256 instances of 50-500 taxa and 2,000-50,000
+  patterns, evaluated in one launch (single slot, as benchmarked); every
+  instance's total against the oracle's, and per-site values of a sample.
+
+Tolerance (FP32 partials, FP64 root and sums): total relative <= 1e-6, per
+site |d lnL| <= 2e-6 max(1, |lnL|).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from aspartik_amd import workloads
+from aspartik_amd.b3 import Batch, Calculator, DatedTree
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+F32_SITE = 2e-6
+F32_TOTAL = 1e-6
+
+
+def _threads(K, P):
+    ncores = len(os.sched_getaffinity(0))
+    return max(1, min(ncores // K, (P + 999) // 1000))
+
+
+def _oracle_mixed(olik, K):
+    site = np.array([olik.site(c) for c in range(K)])
+    mx = site.max(0)
+    return mx + np.log(np.exp(site - mx).sum(0)) + np.log(1.0 / K)
+
+
+def _check_sites(got, ref):
+    err = np.abs(got - ref) / np.maximum(1.0, np.abs(ref))
+    assert err.max() <= F32_SITE, f"max per-site error {err.max():.3e}"
+
+
+def test_gtr_gamma4_128taxa_100000pat(hip_device):
+    wl = workloads.gtr_gamma4(128, 100_000, seed=4)
+    L, P, K = 128, 100_000, wl["K"]
+    print(f"[full-shape] GTR+Gamma4 {L} taxa {P} patterns", flush=True)
+    tree = DatedTree(L, wl["parents"], wl["children"], wl["heights"])
+    model = wl["model"]
+    calc = Calculator.HIP().make(wl["codes"], P, K)
+    calc.set_weights(wl["weights"])
+    calc.set_model(model.q_matrix(), model.frequencies(), True)
+    full = workloads.full_lists(tree)
+    calc.propose_times(*full, workloads.edge_times(tree, full[1]), wl["rates"])
+    total = calc.total()
+    mixed = calc.site_mixed()
+    calc.accept()
+
+    otree = O.Tree(L, wl["parents"], wl["children"], wl["heights"])
+    om = O.Model.gtr(model.rates, model.freqs)
+    olik = O.GammaLikelihood(otree, om, wl["codes"], P, wl["weights"], num_categories=K,
+                             alpha=wl["alpha"], threads_per_category=_threads(K, P),
+                             parallel_categories=True)
+    ref = olik.likelihood()
+    olik.accept()
+    otree.accept()
+    assert abs(total - ref) <= F32_TOTAL * abs(ref), (total, ref)
+    _check_sites(mixed, _oracle_mixed(olik, K))
+
+    # one-edge partial evaluations: node-height moves, each accepted, against
+    # the oracle moved the same way (its own partial update)
+    rng = np.random.default_rng(8)
+    for step in range(4):
+        node = int(rng.integers(L, 2 * L - 2))  # not the root
+        kids = tree.children_of(node)
+        lo = max(tree.heights[kids[0]], tree.heights[kids[1]])
+        hi = tree.heights[tree.parent_of(node)]
+        h = lo + (hi - lo) * float(rng.uniform(0.05, 0.95))
+        tree.set_height(node, h)
+        otree.set_height(node, h)
+        internals, children = tree.partials_lists()
+        calc.propose_times(internals, children, workloads.edge_times(tree, children),
+                           wl["rates"])
+        total = calc.total()
+        ref = olik.likelihood()
+        assert abs(total - ref) <= F32_TOTAL * abs(ref), (step, total, ref)
+        _check_sites(calc.site_mixed(), _oracle_mixed(olik, K))
+        calc.accept()
+        tree.accept()
+        olik.accept()
+        otree.accept()
+    calc.close()
+
+
+def test_batched_hky_gamma4_256_instances(hip_device):
+    wl = workloads.hky_gamma4_batch(count=256)
+    insts = wl["instances"]
+    K, rates = wl["K"], wl["rates"]
+    print(f"[full-shape] batched HKY+Gamma4, {len(insts)} instances", flush=True)
+    ups = []
+    for x in insts:
+        t = DatedTree(x["L"], x["parents"], x["children"], x["heights"])
+        internals, children = workloads.full_lists(t)
+        ups.append((internals, children, workloads.edge_times(t, children), rates))
+    b = Batch([(x["codes"], None) for x in insts], num_categories=K, single_slot=True)
+    for i, x in enumerate(insts):
+        b.set_model(i, x["model"].q_matrix(), x["model"].frequencies(), True)
+    b.propose_times(ups, graph=True)
+    totals = b.totals()
+    b.accept()
+    sample = set(np.random.default_rng(3).choice(len(insts), size=6, replace=False).tolist())
+    site_got = {i: np.array([b.likelihoods(i, k) for k in range(K)]) for i in sample}
+    b.close()
+    worst = 0.0
+    for i, x in enumerate(insts):
+        ot = O.Tree(x["L"], x["parents"], x["children"], x["heights"])
+        om = O.Model.hky(x["kappa"], np.full(4, 0.25))
+        lik = O.GammaLikelihood(ot, om, x["codes"], x["P"], np.ones(x["P"], np.uint32),
+                                num_categories=K, alpha=wl["alpha"],
+                                threads_per_category=_threads(K, x["P"]),
+                                parallel_categories=True)
+        ref = lik.likelihood()
+        rel = abs(totals[i] - ref) / abs(ref)
+        worst = max(worst, rel)
+        assert rel <= F32_TOTAL, (i, totals[i], ref)
+        if i in sample:
+            for k in range(K):
+                _check_sites(site_got[i][k], lik.site(k))
+        del lik, ot
+    print(f"[full-shape] worst instance total rel err {worst:.2e}", flush=True)
