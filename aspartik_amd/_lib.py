@@ -218,6 +218,11 @@ DATA_SIGNATURES = {
     "b3d_deduplicate": (C.c_int, [u8p, C.c_uint32, C.c_uint64, C.c_int, u8p, C.c_uint64, u32p,
                                   np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS"),
                                   C.POINTER(C.c_uint64)]),
+    "b3d_deduplicate_noalloc": (C.c_int, [u8p, C.c_uint32, C.c_uint64, C.c_int, u8p, C.c_uint64,
+                                          u32p,
+                                          np.ctypeslib.ndpointer(np.uint64,
+                                                                 flags="C_CONTIGUOUS"),
+                                          C.POINTER(C.c_uint64)]),
 }
 
 

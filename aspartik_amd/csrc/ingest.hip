@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -308,26 +309,33 @@ __global__ void k_scan_add(uint32_t *__restrict__ out, uint64_t n,
   out[j] += offsets[j / kScanBlock];
 }
 
-hipError_t exclusive_scan(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t s) {
+// Workspace bytes of exclusive_scan_ws over n values: two block-sum arrays
+// per level of the recursion.
+uint64_t scan_ws_bytes(uint64_t n) {
+  const uint64_t blocks = std::max<uint64_t>((n + kScanBlock - 1) / kScanBlock, 1);
+  const uint64_t here = (blocks * 4 * 2 + 255) & ~(uint64_t)255;
+  return blocks > 1 ? here + scan_ws_bytes(blocks) : here;
+}
+
+// Exclusive prefix sum of n u32 values on stream s; ws holds scan_ws_bytes(n).
+hipError_t exclusive_scan_ws(const uint32_t *in, uint32_t *out, uint64_t n, unsigned char *ws,
+                             hipStream_t s) {
   const uint64_t blocks = (n + kScanBlock - 1) / kScanBlock;
-  uint32_t *sums = nullptr, *sums_scan = nullptr;
-  hipError_t e = hipMalloc(&sums, std::max<uint64_t>(blocks, 1) * 4 * 2);
-  if (e != hipSuccess) return e;
-  sums_scan = sums + std::max<uint64_t>(blocks, 1);
+  const uint64_t nb = std::max<uint64_t>(blocks, 1);
+  uint32_t *sums = reinterpret_cast<uint32_t *>(ws);
+  uint32_t *sums_scan = sums + nb;
   hipLaunchKernelGGL(k_scan_block, dim3((uint32_t)blocks), dim3(kScanThreads), 0, s, in, out, n,
                      sums);
-  e = hipGetLastError();
+  hipError_t e = hipGetLastError();
   if (e == hipSuccess && blocks > 1) {
-    e = exclusive_scan(sums, sums_scan, blocks, s);
+    e = exclusive_scan_ws(sums, sums_scan, blocks, ws + ((nb * 4 * 2 + 255) & ~(uint64_t)255), s);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_scan_add, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, out, n,
                          sums_scan);
       e = hipGetLastError();
     }
   }
-  const hipError_t e2 = hipStreamSynchronize(s);
-  (void)hipFree(sums);
-  return e != hipSuccess ? e : e2;
+  return e;
 }
 
 __global__ void k_emit(const uint8_t *__restrict__ msa, uint32_t S, uint64_t N,
@@ -349,6 +357,105 @@ __global__ void k_emit(const uint8_t *__restrict__ msa, uint32_t S, uint64_t N,
   }
 }
 
+__global__ void k_fill32(uint32_t *__restrict__ p, uint64_t n, uint32_t v) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) p[j] = v;
+}
+
+// Device buffers of one deduplication of S sequences x N sites.  leaves, w
+// and s2p are sized by the pattern count P, known after the scan.
+struct DedupBufs {
+  uint64_t cap = 0;  // hash slots: a power of two >= 2N
+  uint8_t *msa = nullptr, *col = nullptr, *leaves = nullptr;
+  uint64_t *h = nullptr, *s2p = nullptr;
+  unsigned long long *keys = nullptr;
+  uint32_t *first = nullptr, *slot = nullptr, *count = nullptr, *isf = nullptr, *pid = nullptr,
+           *rep = nullptr, *w = nullptr, *coll = nullptr;
+  unsigned char *scan_ws = nullptr;
+};
+
+uint64_t hash_slots(uint64_t N) {
+  uint64_t cap = 16;
+  while (cap < 2 * N) cap <<= 1;
+  return cap;
+}
+
+#define DCHK(expr)                                                                         \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      (void)hipGetLastError();                                                             \
+      return host_fail(B3H_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    }                                                                                      \
+  } while (0)
+
+// The kernels of a device deduplication on `stream` (shared by both entry
+// points).  `outputs(P, &b)` makes b.leaves (S x P bytes), b.w (P u32) and
+// b.s2p (N u64) available once the pattern count P is known.  Every buffer
+// that must start cleared is cleared here by a fill kernel.
+template <typename Outputs>
+b3h_status dedup_kernels(DedupBufs &b, const uint8_t *msa, uint32_t S, uint64_t N,
+                         uint8_t *leaves, uint64_t stride, uint32_t *weights, uint64_t *s2p,
+                         uint64_t *np, hipStream_t stream, Outputs outputs) {
+  const uint32_t Sp = (S + 15u) & ~15u;
+  const uint64_t cap = b.cap;
+  DCHK(hipMemcpyAsync(b.msa, msa, (size_t)S * N, hipMemcpyHostToDevice, stream));
+  const dim3 grid((uint32_t)((N + 255) / 256)), block(256);
+  auto fill = [&](uint32_t *p, uint64_t n, uint32_t v) {
+    hipLaunchKernelGGL(k_fill32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, p, n,
+                       v);
+    return hipGetLastError();
+  };
+  bool ok = false;
+  for (uint64_t seed = 0; seed < 4 && !ok; seed++) {
+    DCHK(fill(reinterpret_cast<uint32_t *>(b.keys), cap * 2, 0u));
+    DCHK(fill(b.first, cap, 0xFFFFFFFFu));
+    DCHK(fill(b.count, cap, 0u));
+    DCHK(fill(b.coll, 1, 0u));
+    // the site-major copy does not depend on the seed: written on the first pass
+    hipLaunchKernelGGL(k_col_hash, grid, block, 0, stream, b.msa, S, N, seed, b.h,
+                       seed == 0 ? b.col : nullptr, Sp);
+    DCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_insert, grid, block, 0, stream, b.h, N, b.keys, b.first, cap - 1, b.slot);
+    DCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_verify, grid, block, 0, stream, b.col, Sp, N, b.slot, b.first, b.count,
+                       b.isf, b.rep, b.coll);
+    DCHK(hipGetLastError());
+    uint32_t coll = 0;
+    DCHK(hipMemcpyAsync(&coll, b.coll, 4, hipMemcpyDeviceToHost, stream));
+    DCHK(hipStreamSynchronize(stream));
+    ok = coll == 0;
+  }
+  if (!ok) return host_fail(B3H_ERR_HIP, "column hash collisions under four seeds");
+  DCHK(exclusive_scan_ws(b.isf, b.pid, N, b.scan_ws, stream));
+  uint32_t last[2] = {0, 0};
+  DCHK(hipMemcpyAsync(&last[0], b.pid + (N - 1), 4, hipMemcpyDeviceToHost, stream));
+  DCHK(hipMemcpyAsync(&last[1], b.isf + (N - 1), 4, hipMemcpyDeviceToHost, stream));
+  DCHK(hipStreamSynchronize(stream));
+  const uint64_t P = (uint64_t)last[0] + last[1];
+  b3h_status st = outputs(P, b);
+  if (st) return st;
+  hipLaunchKernelGGL(k_emit, grid, block, 0, stream, b.msa, S, N, b.rep, b.pid, b.slot, b.count,
+                     b.leaves, P, b.w, b.s2p);
+  DCHK(hipGetLastError());
+  DCHK(hipMemcpy2DAsync(leaves, stride, b.leaves, P, P, S, hipMemcpyDeviceToHost, stream));
+  DCHK(hipMemcpyAsync(weights, b.w, P * 4, hipMemcpyDeviceToHost, stream));
+  DCHK(hipMemcpyAsync(s2p, b.s2p, N * 8, hipMemcpyDeviceToHost, stream));
+  DCHK(hipStreamSynchronize(stream));
+  *np = P;
+  return B3H_OK;
+}
+
+b3h_status check_device(int device) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return host_fail(B3H_ERR_NO_DEVICE, "no HIP device visible");
+  }
+  if (device >= ndev) return host_fail(B3H_ERR_NO_DEVICE, "device ordinal out of range");
+  return B3H_OK;
+}
+
 struct DevBufs {
   std::vector<void *> ptrs;
   ~DevBufs() {
@@ -362,90 +469,124 @@ struct DevBufs {
   }
 };
 
-#define DCHK(expr)                                                                         \
-  do {                                                                                     \
-    hipError_t e_ = (expr);                                                                \
-    if (e_ != hipSuccess) {                                                                \
-      if (stream) (void)hipStreamDestroy(stream);                                          \
-      return host_fail(B3H_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
-    }                                                                                      \
-  } while (0)
+struct StreamGuard {
+  hipStream_t s = nullptr;
+  ~StreamGuard() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
 
 b3h_status dedup_device(int device, const uint8_t *msa, uint32_t S, uint64_t N, uint8_t *leaves,
                         uint64_t stride, uint32_t *weights, uint64_t *s2p, uint64_t *np) {
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-    (void)hipGetLastError();
-    return host_fail(B3H_ERR_NO_DEVICE, "no HIP device visible");
-  }
-  if (device >= ndev) return host_fail(B3H_ERR_NO_DEVICE, "device ordinal out of range");
+  b3h_status st = check_device(device);
+  if (st) return st;
   if (N >= 0xFFFFFFFFull) return host_fail(B3H_ERR_UNSUPPORTED, "more than 2^32 - 2 sites");
-  hipStream_t stream = nullptr;
   DCHK(hipSetDevice(device));
-  DCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  StreamGuard sg;
+  DCHK(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
   DevBufs buf;
-  uint64_t cap = 16;
-  while (cap < 2 * N) cap <<= 1;
-  uint8_t *d_msa, *d_leaves, *d_col;
+  DedupBufs b;
+  b.cap = hash_slots(N);
   const uint32_t Sp = (S + 15u) & ~15u;
-  uint64_t *d_h, *d_s2p;
-  unsigned long long *d_keys;
-  uint32_t *d_first, *d_slot, *d_count, *d_isf, *d_pid, *d_rep, *d_w, *d_coll;
-  DCHK(buf.alloc(&d_msa, (size_t)S * N));
-  DCHK(buf.alloc(&d_col, (size_t)Sp * N));
-  DCHK(buf.alloc(&d_h, N * 8));
-  DCHK(buf.alloc(&d_keys, cap * 8));
-  DCHK(buf.alloc(&d_first, cap * 4));
-  DCHK(buf.alloc(&d_count, cap * 4));
-  DCHK(buf.alloc(&d_slot, N * 4));
-  DCHK(buf.alloc(&d_isf, N * 4));
-  DCHK(buf.alloc(&d_pid, N * 4));
-  DCHK(buf.alloc(&d_rep, N * 4));
-  DCHK(buf.alloc(&d_coll, 4));
-  DCHK(hipMemcpyAsync(d_msa, msa, (size_t)S * N, hipMemcpyHostToDevice, stream));
-  const dim3 grid((uint32_t)((N + 255) / 256)), block(256);
-  bool ok = false;
-  for (uint64_t seed = 0; seed < 4 && !ok; seed++) {
-    DCHK(hipMemsetAsync(d_keys, 0, cap * 8, stream));
-    DCHK(hipMemsetAsync(d_first, 0xFF, cap * 4, stream));
-    DCHK(hipMemsetAsync(d_count, 0, cap * 4, stream));
-    DCHK(hipMemsetAsync(d_coll, 0, 4, stream));
-    // the site-major copy does not depend on the seed: written on the first pass
-    hipLaunchKernelGGL(k_col_hash, grid, block, 0, stream, d_msa, S, N, seed, d_h,
-                       seed == 0 ? d_col : nullptr, Sp);
-    DCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_insert, grid, block, 0, stream, d_h, N, d_keys, d_first, cap - 1, d_slot);
-    DCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_verify, grid, block, 0, stream, d_col, Sp, N, d_slot, d_first, d_count,
-                       d_isf, d_rep, d_coll);
-    DCHK(hipGetLastError());
-    uint32_t coll = 0;
-    DCHK(hipMemcpyAsync(&coll, d_coll, 4, hipMemcpyDeviceToHost, stream));
-    DCHK(hipStreamSynchronize(stream));
-    ok = coll == 0;
+  DCHK(buf.alloc(&b.msa, (size_t)S * N));
+  DCHK(buf.alloc(&b.col, (size_t)Sp * N));
+  DCHK(buf.alloc(&b.h, N * 8));
+  DCHK(buf.alloc(&b.keys, b.cap * 8));
+  DCHK(buf.alloc(&b.first, b.cap * 4));
+  DCHK(buf.alloc(&b.count, b.cap * 4));
+  DCHK(buf.alloc(&b.slot, N * 4));
+  DCHK(buf.alloc(&b.isf, N * 4));
+  DCHK(buf.alloc(&b.pid, N * 4));
+  DCHK(buf.alloc(&b.rep, N * 4));
+  DCHK(buf.alloc(&b.coll, 4));
+  DCHK(buf.alloc(&b.scan_ws, scan_ws_bytes(N)));
+  auto outputs = [&](uint64_t P, DedupBufs &bb) -> b3h_status {
+    DCHK(buf.alloc(&bb.leaves, (size_t)S * P));
+    DCHK(buf.alloc(&bb.w, P * 4));
+    DCHK(buf.alloc(&bb.s2p, N * 8));
+    return B3H_OK;
+  };
+  return dedup_kernels(b, msa, S, N, leaves, stride, weights, s2p, np, sg.s, outputs);
+}
+
+// ---- the no-allocation form -------------------------------------------------
+// One scratch array per device, in the library's code object: every buffer
+// of a deduplication is carved from it, so the call allocates nothing.  In
+// host code `dedup_scratch` names a host-side stand-in; the device address
+// of each device's copy comes from hipGetSymbolAddress, taken once per device
+// with that device current, and only that address is used.
+static __device__ __align__(256) unsigned char dedup_scratch[B3D_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+uint64_t align256(uint64_t x) { return (x + 255) & ~(uint64_t)255; }
+
+// Scratch bytes one call needs, with the carve-up: the pattern bytes reuse
+// the site-major copy (dead after k_verify), the weights the site hashes
+// (dead after k_insert) and the site map the hash keys (dead after k_verify).
+uint64_t noalloc_layout(uint32_t S, uint64_t N, unsigned char *base, DedupBufs *b) {
+  const uint32_t Sp = (S + 15u) & ~15u;
+  const uint64_t cap = hash_slots(N);
+  uint64_t off = 0;
+  auto take = [&](uint64_t bytes) {
+    unsigned char *p = base ? base + off : nullptr;
+    off += align256(bytes);
+    return p;
+  };
+  DedupBufs d;
+  d.cap = cap;
+  d.msa = take((uint64_t)S * N);
+  d.col = take((uint64_t)Sp * N);
+  d.h = reinterpret_cast<uint64_t *>(take(N * 8));
+  d.keys = reinterpret_cast<unsigned long long *>(take(cap * 8));
+  d.first = reinterpret_cast<uint32_t *>(take(cap * 4));
+  d.count = reinterpret_cast<uint32_t *>(take(cap * 4));
+  d.slot = reinterpret_cast<uint32_t *>(take(N * 4));
+  d.isf = reinterpret_cast<uint32_t *>(take(N * 4));
+  d.pid = reinterpret_cast<uint32_t *>(take(N * 4));
+  d.rep = reinterpret_cast<uint32_t *>(take(N * 4));
+  d.coll = reinterpret_cast<uint32_t *>(take(4));
+  d.scan_ws = take(scan_ws_bytes(N));
+  d.leaves = d.col;                               // S * P <= Sp * N
+  d.w = reinterpret_cast<uint32_t *>(d.h);        // 4 P <= 8 N
+  d.s2p = reinterpret_cast<uint64_t *>(d.keys);   // 8 N <= 8 cap
+  if (b) *b = d;
+  return off;
+}
+
+b3h_status scratch_of(int device, unsigned char **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device >= kMaxDevices) return host_fail(B3H_ERR_UNSUPPORTED, "device ordinal above 63");
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    DCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(dedup_scratch)));
+    if (!p) return host_fail(B3H_ERR_HIP, "hipGetSymbolAddress returned a null address");
+    g_scratch_ptr[device] = static_cast<unsigned char *>(p);
   }
-  if (!ok) {
-    (void)hipStreamDestroy(stream);
-    return host_fail(B3H_ERR_HIP, "column hash collisions under four seeds");
-  }
-  DCHK(exclusive_scan(d_isf, d_pid, N, stream));
-  uint32_t last_pid = 0, last_flag = 0;
-  DCHK(hipMemcpy(&last_pid, d_pid + (N - 1), 4, hipMemcpyDeviceToHost));
-  DCHK(hipMemcpy(&last_flag, d_isf + (N - 1), 4, hipMemcpyDeviceToHost));
-  const uint64_t P = (uint64_t)last_pid + last_flag;
-  DCHK(buf.alloc(&d_leaves, (size_t)S * P));
-  DCHK(buf.alloc(&d_w, P * 4));
-  DCHK(buf.alloc(&d_s2p, N * 8));
-  hipLaunchKernelGGL(k_emit, grid, block, 0, stream, d_msa, S, N, d_rep, d_pid, d_slot, d_count,
-                     d_leaves, P, d_w, d_s2p);
-  DCHK(hipGetLastError());
-  DCHK(hipMemcpy2DAsync(leaves, stride, d_leaves, P, P, S, hipMemcpyDeviceToHost, stream));
-  DCHK(hipMemcpyAsync(weights, d_w, P * 4, hipMemcpyDeviceToHost, stream));
-  DCHK(hipMemcpyAsync(s2p, d_s2p, N * 8, hipMemcpyDeviceToHost, stream));
-  DCHK(hipStreamSynchronize(stream));
-  (void)hipStreamDestroy(stream);
-  *np = P;
+  *out = g_scratch_ptr[device];
   return B3H_OK;
+}
+
+b3h_status dedup_device_noalloc(int device, const uint8_t *msa, uint32_t S, uint64_t N,
+                                uint8_t *leaves, uint64_t stride, uint32_t *weights,
+                                uint64_t *s2p, uint64_t *np) {
+  const uint64_t need = noalloc_layout(S, N, nullptr, nullptr);
+  if (need > B3D_NOALLOC_SCRATCH_BYTES)
+    return host_fail(B3H_ERR_ARG, "input needs " + std::to_string(need) +
+                                      " bytes of device scratch; the no-allocation form holds " +
+                                      std::to_string((uint64_t)B3D_NOALLOC_SCRATCH_BYTES));
+  b3h_status st = check_device(device);
+  if (st) return st;
+  DCHK(hipSetDevice(device));
+  unsigned char *base = nullptr;
+  if ((st = scratch_of(device, &base))) return st;
+  DedupBufs b;
+  noalloc_layout(S, N, base, &b);
+  // the legacy default stream: creating a stream is not free of allocation
+  auto outputs = [](uint64_t, DedupBufs &) -> b3h_status { return B3H_OK; };
+  return dedup_kernels(b, msa, S, N, leaves, stride, weights, s2p, np, nullptr, outputs);
 }
 
 }  // namespace
@@ -536,6 +677,22 @@ b3h_status b3d_deduplicate(const uint8_t *msa, uint32_t S, uint64_t N, int devic
     if (msa[i] > 15) return host_fail(B3H_ERR_ARG, "not a DnaNucleotide byte at " + std::to_string(i));
   if (device < 0) return dedup_host(msa, S, N, leaves, stride, weights, s2p, np);
   return dedup_device(device, msa, S, N, leaves, stride, weights, s2p, np);
+}
+
+b3h_status b3d_deduplicate_noalloc(const uint8_t *msa, uint32_t S, uint64_t N, int device,
+                                   uint8_t *leaves, uint64_t stride, uint32_t *weights,
+                                   uint64_t *s2p, uint64_t *np) {
+  if (!np) return host_fail(B3H_ERR_ARG, "null pattern count");
+  *np = 0;
+  if (S == 0) return host_fail(B3H_ERR_ARG, "no sequences");
+  if (device < 0) return host_fail(B3H_ERR_ARG, "the no-allocation form runs on a device");
+  if (N == 0) return B3H_OK;
+  if (!msa || !leaves || !weights || !s2p) return host_fail(B3H_ERR_ARG, "null array");
+  if (stride < N) return host_fail(B3H_ERR_ARG, "leaves stride must be at least num_sites");
+  if (N >= 0xFFFFFFFFull) return host_fail(B3H_ERR_UNSUPPORTED, "more than 2^32 - 2 sites");
+  for (uint64_t i = 0; i < (uint64_t)S * N; i++)
+    if (msa[i] > 15) return host_fail(B3H_ERR_ARG, "not a DnaNucleotide byte at " + std::to_string(i));
+  return dedup_device_noalloc(device, msa, S, N, leaves, stride, weights, s2p, np);
 }
 
 }  // extern "C"

@@ -82,21 +82,23 @@ class MSA:
                 and np.array_equal(self._data, other._data))
 
 
-def deduplicate(msa, device: int | None = None):
+def deduplicate(msa, device: int | None = None, no_alloc: bool = False):
     """likelihood/mod.rs:74-132: unique columns in first-occurrence order, their
     counts as weights and each site's pattern; gaps become 0b1111 in the
     pattern bytes.  ``msa`` is an ``MSA`` or a [num_sequences, num_sites] u8
     array of DnaNucleotide bytes.  device None: on the host; an int: on that
-    GPU.  Returns (leaves [num_sequences, num_patterns], weights, sites_to_patterns)."""
+    GPU.  ``no_alloc``: on the GPU through b3d_deduplicate_noalloc, which uses
+    the library's fixed device scratch (64 MiB) instead of allocating.
+    Returns (leaves [num_sequences, num_patterns], weights, sites_to_patterns)."""
     a = msa._data if isinstance(msa, MSA) else np.ascontiguousarray(msa, np.uint8)
     S, N = a.shape
     leaves = np.zeros((S, max(N, 1)), np.uint8)
     weights = np.zeros(max(N, 1), np.uint32)
     s2p = np.zeros(max(N, 1), np.uint64)
     P = C.c_uint64()
-    check(lib().b3d_deduplicate(np.ascontiguousarray(a).reshape(-1), S, N,
-                                -1 if device is None else int(device), leaves.reshape(-1),
-                                leaves.shape[1], weights, s2p, C.byref(P)))
+    fn = lib().b3d_deduplicate_noalloc if no_alloc else lib().b3d_deduplicate
+    check(fn(np.ascontiguousarray(a).reshape(-1), S, N, -1 if device is None else int(device),
+             leaves.reshape(-1), leaves.shape[1], weights, s2p, C.byref(P)))
     P = P.value
     return leaves[:, :P].copy(), weights[:P].copy(), s2p[:N].astype(np.int64)
 

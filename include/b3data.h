@@ -70,6 +70,28 @@ b3h_status b3d_deduplicate(const uint8_t *msa, uint32_t num_sequences, uint64_t 
                            uint32_t *weights, uint64_t *sites_to_patterns,
                            uint64_t *num_patterns);
 
+/*
+ * b3d_deduplicate on a device without allocating: no reference counterpart.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and outputs as b3d_deduplicate with device >= 0 (bit-identical
+ * results; device < 0 is an argument error); it calls no hipMalloc, hipFree
+ * or stream creation and works on the legacy default stream.
+ *   Every buffer comes from one scratch array of B3D_NOALLOC_SCRATCH_BYTES
+ *   (64 MiB) that each device the library runs on holds until the process
+ *   ends.  An input of S sequences x N sites needs about (2 S + 24) bytes
+ *   per site plus 16 bytes per hash slot (a power of two >= 2 N), between
+ *   (2 S + 56) N and (2 S + 88) N bytes (200 sequences: about 140,000 sites);
+ *   larger inputs return B3H_ERR_ARG before any device work.
+ *   Calls that overlap on one device share that device's array and race:
+ *   the caller serialises them (calls on distinct devices are independent).
+ */
+#define B3D_NOALLOC_SCRATCH_BYTES (64ull * 1024ull * 1024ull)
+b3h_status b3d_deduplicate_noalloc(const uint8_t *msa, uint32_t num_sequences,
+                                   uint64_t num_sites, int device, uint8_t *leaves,
+                                   uint64_t leaves_stride, uint32_t *weights,
+                                   uint64_t *sites_to_patterns, uint64_t *num_patterns);
+
 #ifdef __cplusplus
 }
 #endif
