@@ -73,6 +73,11 @@ struct b3h_calc {
   uint32_t *d_counter = nullptr;
   double *d_total = nullptr;  // [2]
   void *d_pmats = nullptr;  // [I][2][K][S*S] float or double
+  // FP32 path: cherries' recorded matrices [2I slots][2][K][5][4] and leaves
+  // [2I]; slot_cherry[s] is 1 while slot s holds a cherry (host mirror)
+  float *d_cmats = nullptr;
+  uint2 *d_cleaves = nullptr;
+  std::vector<uint8_t> slot_cherry;
   DevModel *d_model = nullptr;  // [K]; only [0] is used while one model serves all
   uint8_t *d_classes = nullptr;  // [P] per-pattern category (HeteroLikelihood)
   double *d_ucodon = nullptr;  // MFMA path: [64][4096] eigenvector products
@@ -137,7 +142,8 @@ void release(b3h_calc *c) {
   void *ptrs[] = {c->d_codes, c->d_partials, c->d_exps, c->d_scales, c->d_classes,
                   c->d_scale_sums, c->d_scale_sums_bak, c->d_site, c->d_mixed,
                   c->d_weights, c->d_blocksum, c->d_mix_blocksum,
-                  c->d_counter, c->d_total, c->d_pmats, c->d_model, c->d_ucodon};
+                  c->d_counter, c->d_total, c->d_pmats, c->d_model, c->d_ucodon,
+                  c->d_cmats, c->d_cleaves};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_total) (void)hipHostFree(c->h_total);
@@ -256,6 +262,17 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
   const bool reorder = f32 || codon;
   const uint32_t cap = reorder ? c->sched.capacity : 0;
   std::string err;
+  // FP32: the new slot of every listed node holds a cherry iff both its
+  // children are leaves; stored children in cherry slots are recomputed
+  // (set again before every schedule build, which may clear the flags of
+  // cherries it has to store)
+  uint8_t *cherry = f32 ? c->slot_cherry.data() : nullptr;
+  auto mark_cherries = [&]() {
+    if (!cherry) return;
+    for (uint32_t i = 0; i < n; i++)
+      cherry[Scheduler::slot_of(internals[i], c->L, c->editbuf_next.data())] =
+          (children[2 * i] < c->L && children[2 * i + 1] < c->L) ? 1 : 0;
+  };
   // Partial updates on the FP32 path: a list that forms one chain (entry 0
   // reads two stored children, every later entry the entry before it and
   // one stored child that no earlier entry of this pass wrote) runs in the
@@ -264,7 +281,9 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
   std::vector<uint32_t> &psrc = c->path_srcs;
   psrc.clear();
   if (f32 && n <= 64) {
-    if (!c->sched.build(n, internals, children, c->L, c->editbuf_next.data(), true, 0, &err))
+    mark_cherries();
+    if (!c->sched.build(n, internals, children, c->L, c->editbuf_next.data(), true, 0, &err,
+                        cherry))
       return fail(B3H_ERR_ARG, err);
     path = true;
     for (uint32_t i = 0; i < n && path; i++) {
@@ -285,10 +304,12 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     for (uint32_t k = 0; k < psrc.size() && path; k++)
       if ((psrc[k] & SRC_TAG) == SRC_HBM)
         for (uint32_t j = 0; j + 1 < k && j < n; j++)  // written earlier in this pass
-          if (c->sched.entries[j].out == (psrc[k] & SRC_IDX)) path = false;
+          if (c->sched.entries[j].out == (psrc[k] & SRC_SLOT)) path = false;
   }
+  if (!path) mark_cherries();
   if (!path &&
-      !c->sched.build(n, internals, children, c->L, c->editbuf_next.data(), reorder, cap, &err))
+      !c->sched.build(n, internals, children, c->L, c->editbuf_next.data(), reorder, cap, &err,
+                      cherry))
     return fail(B3H_ERR_ARG, err);
   c->editbuf.swap(c->editbuf_next);
   // every error return below restores the slot selectors, so a failed
@@ -320,7 +341,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     // P = hi + 2^-20 lo (prune_codon.hip)
     _Float16 *dst = reinterpret_cast<_Float16 *>(h + off_m);
     for (uint32_t e = 0; e < n; e++) {
-      const uint32_t orig = c->sched.entries[e].meta >> 8;
+      const uint32_t orig = entry_index(c->sched.entries[e].meta);
       for (uint32_t side = 0; side < 2; side++)
         for (uint32_t cat = 0; cat < K; cat++) {
           const double *src = mats + ((size_t)cat * 2 * n + 2 * orig + side) * SS;
@@ -432,7 +453,7 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
                                 reinterpret_cast<const double *>(h + off_t),
                                 reinterpret_cast<const double *>(h + off_r), c->d_model, mstride,
                                 static_cast<float *>(c->d_pmats), c->stream, h, d,
-                                (uint32_t)((total + 15) / 16)));
+                                (uint32_t)((total + 15) / 16), c->d_cmats, c->d_cleaves));
     HIPCHK(hipEventRecord(c->stage_ev[s], c->stream));  // the last reader of h
     F32Args a{};
     a.entries = d_entries;
@@ -462,6 +483,8 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     a.K = K;
     a.T = c->T;
     a.nslots = std::max<uint32_t>(c->sched.slots_used, 1u);
+    a.cmats = c->d_cmats;
+    a.cleaves = c->d_cleaves;
     if (e0) HIPCHK(hipEventRecord(e0, c->stream));
     // Lists that read children from HBM (partial updates) use the variant
     // that prefetches them, with one pattern per lane (its registers do not
@@ -471,7 +494,8 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
       a.ntiles = path_tiles(c->P, K);
       HIPCHK(launch_prune_path(a, reinterpret_cast<const uint32_t *>(d + off_src), c->stream));
     } else {
-      const bool hbm = c->sched.hbm_sources * 8 > n;
+      // stored cherries are recomputed only by the HBM-prefetch variant
+      const bool hbm = c->sched.hbm_sources * 8 > n || c->sched.cherry_sources > 0;
       const uint32_t ppl = hbm ? 1u : c->PPL;
       const uint32_t tp = 64u * c->T * ppl;
       a.ntiles = (c->P + tp - 1) / tp;
@@ -851,8 +875,14 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
   if ((st = dev_alloc(c, (void **)&c->d_mix_blocksum, (size_t)mix_blocks(P) * 8))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_counter, 64))) return bail(st);
   if ((st = dev_alloc(c, (void **)&c->d_total, 64))) return bail(st);
-  const size_t mat_bytes = codon ? 8192 * 2 : (size_t)S * S * ((f32 || f32_prop) ? 4 : 8);
+  // FP32 4-state matrices as columns plus an all-ones column ([5][4] floats)
+  const size_t mat_bytes = codon ? 8192 * 2 : f32 ? 80 : (size_t)S * S * (f32_prop ? 4 : 8);
   if ((st = dev_alloc(c, &c->d_pmats, (size_t)c->I * 2 * K * mat_bytes))) return bail(st);
+  if (f32) {
+    if ((st = dev_alloc(c, (void **)&c->d_cmats, slots * 2 * K * 20 * 4))) return bail(st);
+    if ((st = dev_alloc(c, (void **)&c->d_cleaves, slots * 8))) return bail(st);
+    c->slot_cherry.assign(slots, 0);
+  }
   if ((st = dev_alloc(c, (void **)&c->d_model, (size_t)K * sizeof(DevModel)))) return bail(st);
   if (codon && (st = dev_alloc(c, (void **)&c->d_ucodon, (size_t)64 * 4096 * 8))) return bail(st);
 
@@ -891,6 +921,9 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
             hipMemset(c->d_classes, 0, P) == hipSuccess &&
             hipMemset(c->d_partials, 0, punits * elem) == hipSuccess;
   if (ok && (f32 || codon || f32_prop)) ok = hipMemset(c->d_exps, 0, punits * 2) == hipSuccess;
+  if (ok && f32)
+    ok = hipMemset(c->d_cmats, 0, slots * 2 * K * 20 * 4) == hipSuccess &&
+         hipMemset(c->d_cleaves, 0, slots * 8) == hipSuccess;
   if (ok && !f32 && !codon && !f32_prop)
     ok = hipMemset(c->d_scales, 0, slots * K * P) == hipSuccess &&
          hipMemset(c->d_scale_sums, 0, (size_t)K * P * 4) == hipSuccess &&

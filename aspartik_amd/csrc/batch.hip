@@ -207,7 +207,7 @@ b3h_status b3h_batch_create(const b3h_config *cfg, uint32_t count, const b3h_ins
     in.bsum_off = bsum;
     bsum += (in.P + 64u * b->T - 1) / (64u * b->T);
     in.pm_off = pm;
-    pm += (size_t)in.I * 2 * K * 16;
+    pm += (size_t)in.I * 2 * K * 20;  // [entry][side][K][5][4]: columns, the fifth all ones
     max_I = std::max(max_I, in.I);
   }
   b->max_items = (uint32_t)bsum;
@@ -410,6 +410,8 @@ b3h_status b3h_batch_propose_times(b3h_batch *b, const b3h_update *ups, uint32_t
     a.seq = 0;
     a.pik = nullptr;
     a.classes = nullptr;
+    a.cmats = nullptr;  // batched instances store every partial, cherries included
+    a.cleaves = nullptr;
     a.ln_inv_k = std::log(1.0 / (double)K);
     a.n = u.n;
     a.P = in.P;

@@ -15,13 +15,18 @@ hipError_t launch_eigen(DevModel *m, hipStream_t s);
 // copy_units > 0: the launch also copies copy_units 16-byte units from
 // copy_src (pinned host memory) to copy_dst, the upload of a staging buffer
 // whose inputs the build then reads in place.
+// cmats / cleaves (4 states, may be null): for every entry whose children
+// are both leaves, its matrices are also written to cmats[out slot]
+// ([2][K][5][4]: columns, the fifth all ones) and its two leaves to
+// cleaves[out slot].
 template <typename T>
 hipError_t launch_pbuild(const Entry *entries, uint32_t n, uint32_t K,
                          uint32_t S, int mode, const double *mats,
                          const double *times, const double *rates,
                          const DevModel *model, uint32_t model_stride, T *out,
                          hipStream_t s, const void *copy_src = nullptr,
-                         void *copy_dst = nullptr, uint32_t copy_units = 0);
+                         void *copy_dst = nullptr, uint32_t copy_units = 0,
+                         T *cmats = nullptr, uint2 *cleaves = nullptr);
 
 struct PbuildDesc {
   const Entry *entries;
@@ -61,6 +66,12 @@ struct F32Args {
   uint32_t n, P, K, T, nslots;
   uint32_t cstride;       // code row stride (a multiple of 64 T)
   uint32_t ntiles;        // tiles of this evaluation (blocksum entries)
+  // Cherries (both children leaves), or null: their partials are then not
+  // stored.  cmats [slots][2][K][5][4]: the two child matrices recorded for a
+  // cherry slot by the P(t) build, as columns plus an all-ones fifth column;
+  // cleaves [slots]: its two leaves.
+  const float *cmats;
+  const uint2 *cleaves;
 };
 hipError_t launch_prune_f32(const F32Args &a, uint32_t ppl, bool hbm, uint32_t blocks,
                             hipStream_t s);

@@ -158,7 +158,9 @@ __device__ inline double p_element(const DevModel *m, int S, double t, int r,
 }
 
 // Build the transition matrices of one propose.  Output layout
-// [entry][side][category][S*S] in the kernels' entry order; `meta >> 8` maps
+// [entry][side][category][S*S] (FP64, row-major) or [entry][side][category]
+// [5][4] (FP32, 4 states: columns and an all-ones fifth column) in the
+// kernels' entry order; entry_index(meta) maps
 // an entry to the caller's list position.  Mode 0 casts host matrices given
 // as [K][2n][S*S] f64; mode 1 builds them from edge_times[2n] x rates[K]
 // (gamma.rs:126-134: time length x (clock rate x category rate)); mode 2
@@ -170,7 +172,8 @@ __global__ void pbuild_kernel(const Entry *entries, uint32_t n, uint32_t K,
                               uint32_t S, int mode, const double *mats,
                               const double *times, const double *rates,
                               const DevModel *model, uint32_t model_stride, T *out,
-                              const uint4 *copy_src, uint4 *copy_dst, uint32_t copy_units) {
+                              const uint4 *copy_src, uint4 *copy_dst, uint32_t copy_units,
+                              T *cmats, uint2 *cleaves) {
   // the upload of the staging buffer, when this launch carries it: the
   // inputs above then point into the pinned host staging buffer itself, and
   // the kernels after this one read the device copy
@@ -187,7 +190,8 @@ __global__ void pbuild_kernel(const Entry *entries, uint32_t n, uint32_t K,
     const uint64_t es = mat / K;
     const uint32_t side = (uint32_t)(es & 1);
     const uint32_t e = (uint32_t)(es >> 1);
-    const uint32_t orig = entries[e].meta >> 8;
+    const Entry en = entries[e];
+    const uint32_t orig = entry_index(en.meta);
     double v;
     if (mode == 0) {
       v = mats[((uint64_t)c * 2 * n + 2ull * orig + side) * SS + el];
@@ -196,7 +200,24 @@ __global__ void pbuild_kernel(const Entry *entries, uint32_t n, uint32_t K,
                                  : times[(uint64_t)c * 2 * n + 2 * orig + side];
       v = p_element(model + c * model_stride, (int)S, t, (int)(el / S), (int)(el % S));
     }
-    out[g] = (T)v;
+    if constexpr (sizeof(T) == 4) {
+      // FP32 (4 states): columns, the fifth all ones (the unknown code),
+      // [entry][side][K][5][4]; a cherry's copy and its leaves are also kept
+      // with its slot for later readers
+      const uint32_t r = el / 4, col = el % 4;
+      T *o = out + mat * 20;
+      o[col * 4 + r] = (T)v;
+      if (col == 0) o[16 + r] = (T)1;
+      if (cmats && (en.left & SRC_TAG) == SRC_TIP && (en.right & SRC_TAG) == SRC_TIP) {
+        T *cm = cmats + (((uint64_t)en.out * 2 + side) * K + c) * 20;
+        cm[col * 4 + r] = (T)v;
+        if (col == 0) cm[16 + r] = (T)1;
+        if (side == 0 && c == 0 && el == 0)
+          cleaves[en.out] = make_uint2(en.left & SRC_IDX, en.right & SRC_IDX);
+      }
+    } else {
+      out[g] = (T)v;
+    }
   }
 }
 
@@ -214,9 +235,13 @@ __global__ void pbuild_batch_kernel(const PbuildDesc *desc, uint32_t K, uint32_t
     const uint64_t es = mat / K;
     const uint32_t side = (uint32_t)(es & 1);
     const uint32_t e = (uint32_t)(es >> 1);
-    const uint32_t orig = d.entries[e].meta >> 8;
+    const uint32_t orig = entry_index(d.entries[e].meta);
     const double t = d.times[2 * orig + side] * d.rates[c];
-    d.out[g] = (float)p_element(d.model, (int)S, t, (int)(el / S), (int)(el % S));
+    // columns, the fifth all ones: [entry][side][K][5][4] (S = 4)
+    const uint32_t r = el / 4, col = el % 4;
+    float *o = d.out + mat * 20;
+    o[col * 4 + r] = (float)p_element(d.model, (int)S, t, (int)r, (int)col);
+    if (col == 0) o[16 + r] = 1.0f;
   }
 }
 
@@ -283,7 +308,7 @@ hipError_t launch_pbuild(const Entry *entries, uint32_t n, uint32_t K,
                          const double *times, const double *rates,
                          const DevModel *model, uint32_t model_stride, T *out,
                          hipStream_t s, const void *copy_src, void *copy_dst,
-                         uint32_t copy_units) {
+                         uint32_t copy_units, T *cmats, uint2 *cleaves) {
   const uint64_t total = (uint64_t)n * 2 * K * S * S;
   uint64_t blocks = std::max<uint64_t>((total + 255) / 256, (copy_units + 255u) / 256u);
   if (blocks > 4096) blocks = 4096;
@@ -291,7 +316,7 @@ hipError_t launch_pbuild(const Entry *entries, uint32_t n, uint32_t K,
   hipLaunchKernelGGL(pbuild_kernel<T>, dim3((uint32_t)blocks), dim3(256), 0, s,
                      entries, n, K, S, mode, mats, times, rates, model, model_stride, out,
                      static_cast<const uint4 *>(copy_src), static_cast<uint4 *>(copy_dst),
-                     copy_units);
+                     copy_units, cmats, cleaves);
   return hipGetLastError();
 }
 
@@ -299,11 +324,13 @@ template hipError_t launch_pbuild<float>(const Entry *, uint32_t, uint32_t,
                                          uint32_t, int, const double *,
                                          const double *, const double *,
                                          const DevModel *, uint32_t, float *,
-                                         hipStream_t, const void *, void *, uint32_t);
+                                         hipStream_t, const void *, void *, uint32_t, float *,
+                                         uint2 *);
 template hipError_t launch_pbuild<double>(const Entry *, uint32_t, uint32_t,
                                           uint32_t, int, const double *,
                                           const double *, const double *,
                                           const DevModel *, uint32_t, double *,
-                                          hipStream_t, const void *, void *, uint32_t);
+                                          hipStream_t, const void *, void *, uint32_t, double *,
+                                          uint2 *);
 
 }  // namespace b3h

@@ -23,7 +23,12 @@
 //    child computed in the same pass is taken from the lane's registers (the
 //    node computed just before) or from a per-wavefront LDS stack instead of
 //    HBM.  Every node is still written to HBM because later partial updates
-//    read it.
+//    read it, except cherries (both children leaves): a later reader
+//    recomputes a cherry from its two leaves' codes and the two matrix
+//    columns the P(t) build recorded for its slot, bit for bit.
+//  - transition matrices are [5][4] floats: four columns and an all-ones
+//    column for the unknown code, so a tip child is one 16-byte LDS read of
+//    column `code` and an internal child four broadcast column reads.
 //  - nothing on a node's critical path waits on memory latency: the whole
 //    update list is staged in LDS at kernel start; transition matrices and
 //    the tile's tip codes stream through a double-buffered LDS chunk whose
@@ -48,41 +53,43 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 
 constexpr uint32_t CH = 4;  // update-list entries per LDS chunk
 
-__device__ __forceinline__ float col_or_one(float m0, float m1, float m2,
-                                            float m3, uint32_t code) {
-  float r = 1.0f;
-  r = code == 0 ? m0 : r;
-  r = code == 1 ? m1 : r;
-  r = code == 2 ? m2 : r;
-  r = code == 3 ? m3 : r;
-  return r;
-}
-
+// A 4-state transition matrix as its four columns: c_k = P[.][k].  In memory
+// every matrix of the FP32 path is [5][4] floats: the four columns and an
+// all-ones fifth column, the value of an unknown tip code (4), so a tip's
+// projection P[.][code] is one 16-byte read at column `code`.
 struct Mat {
-  float4 r0, r1, r2, r3;
+  float4 c0, c1, c2, c3;
 };
 
-// Explicit fused multiply-adds in a fixed order, so every kernel variant
-// (tile shape, patterns per lane) rounds identically.
-__device__ __forceinline__ float dot4(float4 m, float4 y) {
-  return fmaf(m.w, y.w, fmaf(m.z, y.z, fmaf(m.y, y.y, m.x * y.x)));
-}
-
+// P y with explicit fused multiply-adds in a fixed order (per component
+// j: P[j][0] y_0, then + P[j][1] y_1, + P[j][2] y_2, + P[j][3] y_3), so every
+// kernel variant (tile shape, patterns per lane) rounds identically.
 __device__ __forceinline__ float4 apply(const Mat &M, float4 y) {
   float4 o;
-  o.x = dot4(M.r0, y);
-  o.y = dot4(M.r1, y);
-  o.z = dot4(M.r2, y);
-  o.w = dot4(M.r3, y);
+  o.x = fmaf(M.c3.x, y.w, fmaf(M.c2.x, y.z, fmaf(M.c1.x, y.y, M.c0.x * y.x)));
+  o.y = fmaf(M.c3.y, y.w, fmaf(M.c2.y, y.z, fmaf(M.c1.y, y.y, M.c0.y * y.x)));
+  o.z = fmaf(M.c3.z, y.w, fmaf(M.c2.z, y.z, fmaf(M.c1.z, y.y, M.c0.z * y.x)));
+  o.w = fmaf(M.c3.w, y.w, fmaf(M.c2.w, y.z, fmaf(M.c1.w, y.y, M.c0.w * y.x)));
   return o;
 }
 
+// a tip's projection from matrix columns held in registers: column `code`,
+// or all ones for the unknown code 4 (selected per component: a select of
+// whole vectors makes the compiler index a private copy)
+__device__ __forceinline__ float pick(float a0, float a1, float a2, float a3, uint32_t code) {
+  float r = 1.0f;
+  r = code == 0 ? a0 : r;
+  r = code == 1 ? a1 : r;
+  r = code == 2 ? a2 : r;
+  r = code == 3 ? a3 : r;
+  return r;
+}
 __device__ __forceinline__ float4 tip(const Mat &M, uint32_t code) {
   float4 o;
-  o.x = col_or_one(M.r0.x, M.r0.y, M.r0.z, M.r0.w, code);
-  o.y = col_or_one(M.r1.x, M.r1.y, M.r1.z, M.r1.w, code);
-  o.z = col_or_one(M.r2.x, M.r2.y, M.r2.z, M.r2.w, code);
-  o.w = col_or_one(M.r3.x, M.r3.y, M.r3.z, M.r3.w, code);
+  o.x = pick(M.c0.x, M.c1.x, M.c2.x, M.c3.x, code);
+  o.y = pick(M.c0.y, M.c1.y, M.c2.y, M.c3.y, code);
+  o.z = pick(M.c0.z, M.c1.z, M.c2.z, M.c3.z, code);
+  o.w = pick(M.c0.w, M.c1.w, M.c2.w, M.c3.w, code);
   return o;
 }
 
@@ -92,11 +99,66 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) {
 
 __device__ __forceinline__ Mat load_mat(const float4 *p) {
   Mat m;
-  m.r0 = p[0];
-  m.r1 = p[1];
-  m.r2 = p[2];
-  m.r3 = p[3];
+  m.c0 = p[0];
+  m.c1 = p[1];
+  m.c2 = p[2];
+  m.c3 = p[3];
   return m;
+}
+
+// x <- x / 2^e with e the binary exponent of max_j x_j (exact); returns e
+__device__ __forceinline__ int normalise(float4 &v) {
+  const float mx = fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w));
+  const int ex = (mx > 0.0f) ? __builtin_amdgcn_frexp_expf(mx) : 0;
+  v.x = __builtin_amdgcn_ldexpf(v.x, -ex);
+  v.y = __builtin_amdgcn_ldexpf(v.y, -ex);
+  v.z = __builtin_amdgcn_ldexpf(v.z, -ex);
+  v.w = __builtin_amdgcn_ldexpf(v.w, -ex);
+  return ex;
+}
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define B3H_CONSTANT __attribute__((address_space(4)))
+#else
+#define B3H_CONSTANT
+#endif
+typedef const B3H_CONSTANT float4 ConstF4;
+typedef const B3H_CONSTANT uint4 ConstU4;
+typedef const B3H_CONSTANT uint2 ConstU2;
+typedef const B3H_CONSTANT uint32_t ConstU32;
+
+__device__ __forceinline__ Mat load_mat_uniform(ConstF4 *p) {
+  Mat m;
+  m.c0 = p[0];
+  m.c1 = p[1];
+  m.c2 = p[2];
+  m.c3 = p[3];
+  return m;
+}
+
+// A cherry slot (both children leaves; its partial is not stored) is
+// recomputed from its two leaves' codes and the two child matrices the P(t)
+// build recorded for the slot as columns: cmats [slot][side][K][5][4], column
+// 4 all ones (the unknown code), so a lane loads the one column its code
+// selects -- the values tip() selects, so the bits are those the pass that
+// computed the cherry had.
+__device__ __forceinline__ float4 cherry_col(const float *cmats, uint32_t slot, uint32_t side,
+                                             uint32_t K, uint32_t c, uint32_t code) {
+  const float4 *m = reinterpret_cast<const float4 *>(cmats);
+  return m[((size_t)(slot * 2 + side) * K + c) * 5 + code];
+}
+// codes packed a | b << 8
+__device__ __forceinline__ float4 cherry_value(const float *cmats, uint32_t slot, uint32_t K,
+                                               uint32_t c, uint32_t codes, int &e) {
+  const float4 l = cherry_col(cmats, slot, 0, K, c, codes & 0xFFu);
+  const float4 r = cherry_col(cmats, slot, 1, K, c, (codes >> 8) & 0xFFu);
+  float4 v;
+  v.x = l.x * r.x;
+  v.y = l.y * r.y;
+  v.z = l.z * r.z;
+  v.w = l.w * r.w;
+  e = exp_clamp(normalise(v));
+  return v;
 }
 
 }  // namespace
@@ -106,7 +168,7 @@ size_t prune_f32_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n,
   const size_t W = (size_t)K * T;
   const size_t TP = 64 * (size_t)T * ppl;
   size_t b = (size_t)n * 16;               // entries
-  b += 2 * ((size_t)CH * 2 * K * 64);      // matrix chunks
+  b += 2 * ((size_t)CH * 2 * K * 80);      // matrix chunks ([5][4] floats each)
   b += 2 * ((size_t)CH * 2 * TP);          // code chunks
   b = (b + 15) & ~(size_t)15;
   // the stacks (float4 + int16 per lane and slot) are dead once the node loop
@@ -147,7 +209,8 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
   }
   const uint32_t ns = a.nslots;
   const size_t KP = (size_t)K * a.P;
-  const uint32_t MATC = CH * 2 * K * 16;  // floats per matrix chunk
+  const uint8_t *codes8 = reinterpret_cast<const uint8_t *>(a.codes32);
+  const uint32_t MATC = CH * 2 * K * 20;  // floats per matrix chunk
   const uint32_t CODEC = CH * 2 * TP;     // bytes per code chunk
 
   // ---- LDS carve-up (mirrors prune_f32_lds_bytes) ----
@@ -182,8 +245,8 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
   do {                                                                         \
     const uint32_t base_ = (ck) * CH;                                          \
     const uint32_t cnt_ = min(CH, n - base_);                                  \
-    const uint32_t nm_ = cnt_ * 2 * K * 16;                                    \
-    const float *gm_ = a.pmats + (size_t)base_ * 2 * K * 16;                   \
+    const uint32_t nm_ = cnt_ * 2 * K * 20;                                    \
+    const float *gm_ = a.pmats + (size_t)base_ * 2 * K * 20;                   \
     _Pragma("unroll") for (uint32_t r_ = 0; r_ < MAT_REGS; r_++) {             \
       const uint32_t q_ = tid + r_ * nthr;                                     \
       mreg[r_] = q_ < nm_ ? gm_[q_] : 0.0f;                                    \
@@ -209,7 +272,7 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
   do {                                                                         \
     const uint32_t base_ = (ck) * CH;                                          \
     const uint32_t cnt_ = min(CH, n - base_);                                  \
-    const uint32_t nm_ = cnt_ * 2 * K * 16;                                    \
+    const uint32_t nm_ = cnt_ * 2 * K * 20;                                    \
     const uint32_t nc_ = cnt_ * 2 * TPW;                                       \
     float *ms_ = mat_s + (size_t)(buf) * MATC;                                 \
     uint32_t *cs_ = reinterpret_cast<uint32_t *>(code_s + (size_t)(buf) * CODEC); \
@@ -236,24 +299,33 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
     hl[q] = hr[q] = make_float4(0.f, 0.f, 0.f, 0.f);
     hel[q] = her[q] = 0;
   }
+  // a stored child: its partial and exponent, or for a cherry the two codes
+  // (packed in the exponent register) it is recomputed from
+  auto fetch_hbm = [&](uint32_t sv, float4 *hv, int *he) {
+    const uint32_t slot = sv & SRC_SLOT;
+    if (sv & SRC_CHERRY) {
+      const uint2 lv = ((ConstU2 *)a.cleaves)[slot];
+#pragma unroll
+      for (int q = 0; q < PPL; q++) {
+        hv[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        he[q] = (int)codes8[(size_t)lv.x * a.cstride + pp[q]] |
+                ((int)codes8[(size_t)lv.y * a.cstride + pp[q]] << 8);
+      }
+    } else {
+      const size_t o = (size_t)slot * KP + (size_t)c * a.P;
+#pragma unroll
+      for (int q = 0; q < PPL; q++) {
+        hv[q] = a.partials[o + pp[q]];
+        he[q] = exp_unpack(a.exps[o + pp[q]]);
+      }
+    }
+  };
 #define B3H_FETCH_HBM(i)                                                       \
   do {                                                                         \
     const uint4 e_ = ent_s[(i)];                                               \
     const uint32_t ls_ = uni(e_.y), rs_ = uni(e_.z);                           \
-    if ((ls_ & SRC_TAG) == SRC_HBM) {                                          \
-      const size_t o_ = (size_t)(ls_ & SRC_IDX) * KP + (size_t)c * a.P;        \
-      _Pragma("unroll") for (int q = 0; q < PPL; q++) {                        \
-        hl[q] = a.partials[o_ + pp[q]];                                        \
-        hel[q] = exp_unpack(a.exps[o_ + pp[q]]);                               \
-      }                                                                        \
-    }                                                                          \
-    if ((rs_ & SRC_TAG) == SRC_HBM) {                                          \
-      const size_t o_ = (size_t)(rs_ & SRC_IDX) * KP + (size_t)c * a.P;        \
-      _Pragma("unroll") for (int q = 0; q < PPL; q++) {                        \
-        hr[q] = a.partials[o_ + pp[q]];                                        \
-        her[q] = exp_unpack(a.exps[o_ + pp[q]]);                               \
-      }                                                                        \
-    }                                                                          \
+    if ((ls_ & SRC_TAG) == SRC_HBM) fetch_hbm(ls_, hl, hel);                   \
+    if ((rs_ & SRC_TAG) == SRC_HBM) fetch_hbm(rs_, hr, her);                   \
   } while (0)
   if (HBM) B3H_FETCH_HBM(0u);
 
@@ -289,57 +361,65 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
         }
         if (i + 1 < n) B3H_FETCH_HBM(i + 1);
       }
-      const Mat ML = load_mat(ms + ((size_t)(j * 2 + 0) * K + c) * 4);
-      const Mat MR = load_mat(ms + ((size_t)(j * 2 + 1) * K + c) * 4);
+      // one child: a tip is its matrix column `code` read from LDS (the
+      // fifth column, all ones, for an unknown code); any other child is
+      // projected with the four columns
+      auto child = [&](uint32_t src, uint32_t side, const float4 *cl_, const int *cel_,
+                       float4 *v, int *ee) {
+        const float4 *mc = ms + ((size_t)(j * 2 + side) * K + c) * 5;
+        const uint32_t tag = src & SRC_TAG, idx = src & SRC_IDX;
+        if (tag == SRC_TIP) {
+#pragma unroll
+          for (int q = 0; q < PPL; q++) {
+            v[q] = mc[cs[(j * 2 + side) * TP + t * 64 * PPL + q * 64 + lane]];
+            ee[q] = 0;
+          }
+          return;
+        }
+        const Mat M = load_mat(mc);
+        if (tag == SRC_PREV) {
+#pragma unroll
+          for (int q = 0; q < PPL; q++) {
+            v[q] = apply(M, x[q]);
+            ee[q] = E[q];
+          }
+        } else if (tag == SRC_LDS) {
+#pragma unroll
+          for (int q = 0; q < PPL; q++) {
+            v[q] = apply(M, stk[(idx * PPL + q) * 64 + lane]);
+            ee[q] = exp_unpack(stke[(idx * PPL + q) * 64 + lane]);
+          }
+        } else if (HBM) {
+          // stored cherries come only with the HBM-prefetch variant (host rule)
+          if (src & SRC_CHERRY) {
+#pragma unroll
+            for (int q = 0; q < PPL; q++)
+              v[q] = apply(M, cherry_value(a.cmats, src & SRC_SLOT, K, c, (uint32_t)cel_[q], ee[q]));
+          } else {
+#pragma unroll
+            for (int q = 0; q < PPL; q++) {
+              v[q] = apply(M, cl_[q]);
+              ee[q] = cel_[q];
+            }
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < PPL; q++) {
+            const size_t o = (size_t)(idx & SRC_SLOT) * KP + (size_t)c * a.P + pp[q];
+            v[q] = apply(M, a.partials[o]);
+            ee[q] = exp_unpack(a.exps[o]);
+          }
+        }
+      };
       float4 lv[PPL], rv[PPL];
       int le[PPL], re[PPL];
-      {
-        const uint32_t tag = lsrc & SRC_TAG, idx = lsrc & SRC_IDX;
-#pragma unroll
-        for (int q = 0; q < PPL; q++) {
-          if (tag == SRC_TIP) {
-            lv[q] = tip(ML, cs[(j * 2 + 0) * TP + t * 64 * PPL + q * 64 + lane]);
-            le[q] = 0;
-          } else if (tag == SRC_PREV) {
-            lv[q] = apply(ML, x[q]);
-            le[q] = E[q];
-          } else if (tag == SRC_LDS) {
-            lv[q] = apply(ML, stk[(idx * PPL + q) * 64 + lane]);
-            le[q] = exp_unpack(stke[(idx * PPL + q) * 64 + lane]);
-          } else if (HBM) {
-            lv[q] = apply(ML, cl[q]);
-            le[q] = cel[q];
-          } else {
-            const size_t o = (size_t)idx * KP + (size_t)c * a.P + pp[q];
-            lv[q] = apply(ML, a.partials[o]);
-            le[q] = exp_unpack(a.exps[o]);
-          }
-        }
-      }
-      {
-        const uint32_t tag = rsrc & SRC_TAG, idx = rsrc & SRC_IDX;
-#pragma unroll
-        for (int q = 0; q < PPL; q++) {
-          if (tag == SRC_TIP) {
-            rv[q] = tip(MR, cs[(j * 2 + 1) * TP + t * 64 * PPL + q * 64 + lane]);
-            re[q] = 0;
-          } else if (tag == SRC_PREV) {
-            rv[q] = apply(MR, x[q]);
-            re[q] = E[q];
-          } else if (tag == SRC_LDS) {
-            rv[q] = apply(MR, stk[(idx * PPL + q) * 64 + lane]);
-            re[q] = exp_unpack(stke[(idx * PPL + q) * 64 + lane]);
-          } else if (HBM) {
-            rv[q] = apply(MR, cr[q]);
-            re[q] = cer[q];
-          } else {
-            const size_t o = (size_t)idx * KP + (size_t)c * a.P + pp[q];
-            rv[q] = apply(MR, a.partials[o]);
-            re[q] = exp_unpack(a.exps[o]);
-          }
-        }
-      }
+      child(lsrc, 0, cl, cel, lv, le);
+      child(rsrc, 1, cr, cer, rv, re);
       const uint32_t push = meta & 0xFFu;
+      // a cherry's partial is not stored when its matrices are recorded,
+      // unless this pass reads it back from memory (META_KEEP)
+      const bool store = !(a.cmats && (lsrc & SRC_TAG) == SRC_TIP &&
+                           (rsrc & SRC_TAG) == SRC_TIP) || (meta & META_KEEP);
 #pragma unroll
       for (int q = 0; q < PPL; q++) {
         float4 v;
@@ -355,7 +435,7 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
         v.w = __builtin_amdgcn_ldexpf(v.w, -ex);
         x[q] = v;
         E[q] = exp_clamp(le[q] + re[q] + ex);
-        if (active[q]) {
+        if (active[q] && store) {
           const size_t o = (size_t)out * KP + (size_t)c * a.P + p[q];
 #if B3H_NT_STORE
           const f4v xv = {v.x, v.y, v.z, v.w};
@@ -526,24 +606,7 @@ __global__ __launch_bounds__(512) void prune_f32_batch_kernel(const F32Args *ins
 // carries the chain changes no bit -- and values match the other variants.
 constexpr uint32_t kPathSeg = 4;
 
-#if defined(__HIP_DEVICE_COMPILE__)
-#define B3H_CONSTANT __attribute__((address_space(4)))
-#else
-#define B3H_CONSTANT
-#endif
-typedef const B3H_CONSTANT float4 ConstF4;
-typedef const B3H_CONSTANT uint4 ConstU4;
-typedef const B3H_CONSTANT uint32_t ConstU32;
 typedef const B3H_CONSTANT F32Args ConstF32Args;
-
-__device__ __forceinline__ Mat load_mat_uniform(ConstF4 *p) {
-  Mat m;
-  m.r0 = p[0];
-  m.r1 = p[1];
-  m.r2 = p[2];
-  m.r3 = p[3];
-  return m;
-}
 
 __global__ __launch_bounds__(1024) void prune_path_kernel(F32Args arg, const uint32_t *srcs) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -569,11 +632,17 @@ __global__ __launch_bounds__(1024) void prune_path_kernel(F32Args arg, const uin
   ConstF4 *mats = (ConstF4 *)a.pmats;
   ConstU32 *tab = (ConstU32 *)srcs;
 
-  // a stored child: a partial (value, exponent) or a leaf's code (in `e`)
+  // a stored child: a partial (value, exponent), a leaf's code (in `e`), or
+  // a cherry recomputed from its two leaves' codes (value, exponent)
   auto load_src = [&](uint32_t sv, float4 &v, int &e) {
     const uint32_t idx = sv & SRC_IDX;
-    if ((sv & SRC_TAG) == SRC_HBM) {
-      const size_t o = (size_t)idx * KP + (size_t)c * a.P + pp;
+    if ((sv & SRC_TAG) == SRC_HBM && (sv & SRC_CHERRY)) {
+      const uint2 lv = ((ConstU2 *)a.cleaves)[sv & SRC_SLOT];
+      const uint32_t cc = (uint32_t)codes8[(size_t)lv.x * a.cstride + pp] |
+                          ((uint32_t)codes8[(size_t)lv.y * a.cstride + pp] << 8);
+      v = cherry_value(a.cmats, sv & SRC_SLOT, K, c, cc, e);
+    } else if ((sv & SRC_TAG) == SRC_HBM) {
+      const size_t o = (size_t)(idx & SRC_SLOT) * KP + (size_t)c * a.P + pp;
       v = a.partials[o];
       e = exp_unpack(a.exps[o]);
     } else {
@@ -618,8 +687,8 @@ __global__ __launch_bounds__(1024) void prune_path_kernel(F32Args arg, const uin
         const uint4 e4 = ents[i];
         // side of the chain (entry 0: its left child, source 0)
         const uint32_t cs = (i == 0 || (e4.y & SRC_TAG) == SRC_PREV) ? 0u : 1u;
-        const Mat MC = load_mat_uniform(mats + ((size_t)(i * 2 + cs) * K + c) * 4);
-        const Mat MS = load_mat_uniform(mats + ((size_t)(i * 2 + (cs ^ 1u)) * K + c) * 4);
+        const Mat MC = load_mat_uniform(mats + ((size_t)(i * 2 + cs) * K + c) * 5);
+        const Mat MS = load_mat_uniform(mats + ((size_t)(i * 2 + (cs ^ 1u)) * K + c) * 5);
         float4 cv, sv4;
         int ce, se;
         if (i == 0) project(MC, s0, v0, e0, cv, ce);
@@ -641,7 +710,10 @@ __global__ __launch_bounds__(1024) void prune_path_kernel(F32Args arg, const uin
         v.w = __builtin_amdgcn_ldexpf(v.w, -ex);
         x = v;
         E = exp_clamp(ce + se + ex);
-        if (active) {
+        // entry 0 may be a cherry: its matrices are recorded, its partial not
+        const bool store = !(i == 0 && a.cmats && (s0 & SRC_TAG) == SRC_TIP &&
+                             (sv[j] & SRC_TAG) == SRC_TIP) || (e4.w & META_KEEP);
+        if (active && store) {
           const size_t o = (size_t)e4.x * KP + (size_t)c * a.P + p;
 #if B3H_NT_STORE
           const f4v xv = {v.x, v.y, v.z, v.w};

@@ -419,7 +419,7 @@ __global__ __launch_bounds__(256) void pbuild_codon_kernel(const Entry *entries,
     double v = 0.0;
     if (mat < nmat && q < S) {
       const uint32_t c = mat % K, es = mat / K, side = es & 1u, e = es >> 1;
-      const uint32_t orig = entries[e].meta >> 8;
+      const uint32_t orig = entry_index(entries[e].meta);
       v = exp(model->eval[q] * (times[2 * orig + side] * rates[c]));
     }
     E[mi][q] = v;

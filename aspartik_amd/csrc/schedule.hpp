@@ -32,6 +32,7 @@ struct Scheduler {
   uint32_t capacity = 16;  // LDS stack slots per wavefront
   uint32_t slots_used = 0;
   uint32_t hbm_sources = 0;  // children read from HBM
+  uint32_t cherry_sources = 0;  // of those, stored cherries (SRC_CHERRY)
   std::vector<Entry> entries;
 
   std::vector<int32_t> pos;    // node id -> list index, -1 when absent
@@ -41,6 +42,7 @@ struct Scheduler {
   std::vector<uint8_t> consumed;
   std::vector<uint8_t> pushed;
   std::vector<uint32_t> base;
+  std::vector<uint32_t> outpos;  // list index -> position in entries
 
   // partial slot of an internal node: two slots per node selected by the
   // EditBuf active bit, or one slot per node when editbuf is null
@@ -49,9 +51,27 @@ struct Scheduler {
     return 2u * (node - L) + (editbuf[node - L] & 1u);
   }
 
+  // slot_cherry (may be null): per partial slot, 1 when it holds a cherry
+  // (both children leaves), whose partial the FP32 path does not store.  A
+  // stored child in such a slot gets SRC_CHERRY.  A cherry of this pass that
+  // its parent reads back from memory (LDS stack overflow) is stored after
+  // all: its entry gets META_KEEP and its slot is no longer a cherry slot.
   bool build(uint32_t n, const uint32_t *internals, const uint32_t *children,
              uint32_t L, const uint8_t *editbuf, bool reorder, uint32_t cap,
-             std::string *err) {
+             std::string *err, uint8_t *slot_cherry = nullptr) {
+    outpos.assign(n, 0);
+    // k: the child's list index when it is computed in this pass, else -1
+    auto hbm_src = [&](uint32_t ch, int32_t k) {
+      const uint32_t s = slot_of(ch, L, editbuf);
+      if (!slot_cherry || !slot_cherry[s]) return SRC_HBM | s;
+      if (k >= 0) {
+        slot_cherry[s] = 0;
+        entries[outpos[k]].meta |= META_KEEP;
+        return SRC_HBM | s;
+      }
+      cherry_sources++;
+      return SRC_HBM | SRC_CHERRY | s;
+    };
     const uint32_t N = 2 * L - 1;
     pos.assign(N, -1);
     for (uint32_t i = 0; i < n; i++) {
@@ -87,17 +107,19 @@ struct Scheduler {
     entries.resize(n);
     slots_used = 0;
     hbm_sources = 0;
+    cherry_sources = 0;
     if (!reorder) {
       for (uint32_t i = 0; i < n; i++) {
         Entry &e = entries[i];
+        outpos[i] = i;
         e.out = slot_of(internals[i], L, editbuf);
+        e.meta = NO_PUSH | (i << 8);
         for (int s = 0; s < 2; s++) {
           const uint32_t ch = children[2 * i + s];
-          const uint32_t src = ch < L ? (SRC_TIP | ch) : (SRC_HBM | slot_of(ch, L, editbuf));
+          const uint32_t src = ch < L ? (SRC_TIP | ch) : hbm_src(ch, kid[2 * i + s]);
           hbm_sources += ch >= L;
           (s ? e.right : e.left) = src;
         }
-        e.meta = NO_PUSH | (i << 8);
       }
       return true;
     }
@@ -161,8 +183,10 @@ struct Scheduler {
         base[v] = b0;
         const bool push = st[top].push && b0 < cap;
         pushed[v] = push ? 1 : 0;
+        outpos[v] = out_i;
         Entry &e = entries[out_i++];
         e.out = slot_of(internals[v], L, editbuf);
+        e.meta = (push ? b0 : NO_PUSH) | ((uint32_t)v << 8);
         for (int s = 0; s < 2; s++) {
           const uint32_t ch = children[2 * v + s];
           const int32_t k = kid[2 * v + s];
@@ -170,11 +194,10 @@ struct Scheduler {
           if (ch < L) src = SRC_TIP | ch;
           else if (k >= 0 && k == k2) src = SRC_PREV;
           else if (k >= 0 && pushed[k]) src = SRC_LDS | base[k];
-          else src = SRC_HBM | slot_of(ch, L, editbuf);
+          else src = hbm_src(ch, k);
           hbm_sources += (src & SRC_TAG) == SRC_HBM;
           (s ? e.right : e.left) = src;
         }
-        e.meta = (push ? b0 : NO_PUSH) | ((uint32_t)v << 8);
         if (push && b0 + 1 > slots_used) slots_used = b0 + 1;
         st.pop_back();
       }

@@ -92,41 +92,56 @@ def algorithmic_bytes(internals, children, num_leaves, root, K, P, states=4, ele
     return B + 4 * P + 8
 
 
-def store_bytes(internals, children, num_leaves, K, P, cherries_stored=True, unit=18,
+def store_bytes(internals, children, num_leaves, K, P, tree_children=None, unit=18,
                 reread_in_pass=False, epilogue=True):
     """HBM bytes one evaluation of this update list must move with this
     library's storage (the roofline basis of the pruning kernels):
       - every listed node's partial written once: ``unit`` bytes per pattern
         and category (FP32: a float4 and an int16 exponent, 18 B; codon FP16:
-        64 padded states and an exponent, 130 B).  A cherry, both of whose
-        children are leaves, is written only when ``cherries_stored``;
+        64 padded states and an exponent, 130 B).  With ``tree_children``
+        (the tree's [L-1, 2] child array; the FP32 path) a cherry, both of
+        whose children are leaves, is not written;
       - every stored child that the pass does not compute read once (``unit``
-        bytes per pattern and category; a leaf: its code byte per pattern);
+        bytes per pattern and category; a leaf: its code byte per pattern; a
+        cherry, with ``tree_children``: its two leaves' code bytes, from which
+        it is recomputed);
       - with ``reread_in_pass`` (the codon kernel, which has no LDS stack), the
         first-evaluated internal child of a node whose children are both
         internal is read back as well;
       - with ``epilogue``, the pattern weights read (4 B) and the site values
-        written (8 B per category plus 8 B mixed) per pattern.
+        written (8 B per category plus 8 B mixed) per pattern; without it,
+        only the per-category site values.
     A child computed in the same pass is otherwise consumed from registers or
-    LDS and not read back."""
+    LDS and not read back.  The few hundred bytes of transition matrices and
+    update list are not counted."""
     internals = np.asarray(internals).reshape(-1)
     children = np.asarray(children).reshape(-1, 2)
+    L = num_leaves
     listed = set(int(n) for n in internals)
+
+    def cherry(n):
+        if tree_children is None:
+            return False
+        a, b = tree_children[n - L]
+        return a < L and b < L
+
     part = unit * K * P
     B = 0
     leaves_read = set()
     for n, ch in zip(internals, children):
-        cherry = ch[0] < num_leaves and ch[1] < num_leaves
-        if cherries_stored or not cherry:
+        if not (tree_children is not None and ch[0] < L and ch[1] < L):
             B += part
         both_internal = True
         for c in ch:
             c = int(c)
-            if c < num_leaves:
+            if c < L:
                 leaves_read.add(c)
                 both_internal = False
             elif c not in listed:
-                B += part
+                if cherry(c):
+                    leaves_read.update(int(x) for x in tree_children[c - L])
+                else:
+                    B += part
                 both_internal = False
         if reread_in_pass and both_internal:
             B += part

@@ -598,7 +598,7 @@ def bench_hetero(args):
     total = calc.total()
     calc.close()
 
-    B = workloads.store_bytes(full[0], full[1], L, K, P)
+    B = workloads.store_bytes(full[0], full[1], L, K, P, tree_children=tree.children)
     achieved = B / (kernel_ms * 1e-3) / 1e9
     cpu = err = None
     if not args.no_cpu:
@@ -877,6 +877,10 @@ def main():
     pk_ms, pk_n = calc.profile_read()
     calc.profile_enable(False)
     mean_path = float(np.mean([pl[0].size for pl in plists]))
+    # mean bytes of the one-edge updates (stored siblings read, path written)
+    B_part = float(np.mean([workloads.store_bytes(pl[0], pl[1], L, K, P,
+                                                  tree_children=tree.children)
+                            for pl in plists]))
     log(f"[bench] fp32 partial: {nparts / dt_part:.1f} evals/s (mean path {mean_path:.1f} nodes)")
     # restore the full state for the error check
     calc.propose_times(*full, times_full, rates)
@@ -902,7 +906,7 @@ def main():
     # bytes the evaluation must move with this storage (the roofline basis),
     # the reference algorithm's byte count (every child read from memory) and
     # flops of one full evaluation
-    B = workloads.store_bytes(full[0], full[1], L, K, P)
+    B = workloads.store_bytes(full[0], full[1], L, K, P, tree_children=tree.children)
     B_ref = workloads.algorithmic_bytes(full[0], full[1], L, root, K, P)
     F = workloads.algorithmic_flops(full[0], full[1], L, K, P)
     achieved = B / (kernel_ms * 1e-3) / 1e9
@@ -998,6 +1002,8 @@ def main():
         "pipelined_evals_per_s": args.steps / dt_pipe,
         "partial_evals_per_s": nparts / dt_part,
         "partial_kernel_us": pk_ms / max(1, pk_n) * 1e3,
+        "partial_algorithmic_bytes": B_part,
+        "partial_GBps": B_part / (pk_ms / max(1, pk_n) * 1e-3) / 1e9,
         "partial_mean_path_nodes": mean_path,
         "partial_updates_per_s": evals_per_s * len(full[0]) * P * K,
         "fp64_plain": f64,

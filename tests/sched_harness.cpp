@@ -60,6 +60,14 @@ int main(int argc, char **argv) {
       if (a > b) std::swap(a, b);
       return f(self(self, a), self(self, b), n);
     };
+    // the FP32 path's cherry rule: a cherry slot's value is never stored;
+    // readers must carry SRC_CHERRY exactly when the slot holds a cherry and
+    // recompute it from its recorded leaves
+    std::vector<uint8_t> slot_cherry(2 * (L - 1), 0);
+    struct Rec {
+      uint32_t a, b, node;
+    };
+    std::vector<Rec> crec(2 * (L - 1), Rec{0, 0, 0});
     Scheduler s;
     s.capacity = 1 + rng() % 6;
     for (int step = 0; step < 8; step++) {
@@ -73,8 +81,11 @@ int main(int argc, char **argv) {
       const uint32_t n = or_tree_partials_lists(tr, ints.data(), ch.data());
       std::vector<uint8_t> e2 = eb;
       for (uint32_t i = 0; i < n; i++) or_editbuf_set_edited(e2.data(), ints[i] - L);
+      for (uint32_t i = 0; i < n; i++)
+        slot_cherry[Scheduler::slot_of(ints[i], L, e2.data())] = ch[2 * i] < L && ch[2 * i + 1] < L;
       std::string err;
-      if (!s.build(n, ints.data(), ch.data(), L, e2.data(), true, s.capacity, &err)) {
+      if (!s.build(n, ints.data(), ch.data(), L, e2.data(), true, s.capacity, &err,
+                   slot_cherry.data())) {
         printf("FAIL build: %s\n", err.c_str());
         return 1;
       }
@@ -88,7 +99,7 @@ int main(int argc, char **argv) {
       std::vector<uint8_t> seen(n, 0);
       for (uint32_t i = 0; i < n; i++) {
         const Entry &e = s.entries[i];
-        const uint32_t orig = e.meta >> 8;
+        const uint32_t orig = entry_index(e.meta);
         if (orig >= n || seen[orig]) {
           printf("FAIL entry order\n");
           return 1;
@@ -105,8 +116,14 @@ int main(int argc, char **argv) {
             if (idx >= s.slots_used) { printf("FAIL stack read out of range\n"); exit(1); }
             return stk[idx];
           }
-          if (idx != 2 * (chid - L) + (eb[chid - L] & 1u)) { printf("FAIL hbm slot\n"); exit(1); }
-          return hbm[idx];
+          const uint32_t slot = src & SRC_SLOT;
+          if (slot != 2 * (chid - L) + (eb[chid - L] & 1u)) { printf("FAIL hbm slot\n"); exit(1); }
+          if ((src & SRC_CHERRY) != 0 != (slot_cherry[slot] != 0)) {
+            printf("FAIL cherry tag\n");
+            exit(1);
+          }
+          if (src & SRC_CHERRY) return f(leafv[crec[slot].a], leafv[crec[slot].b], crec[slot].node);
+          return hbm[slot];
         };
         const double a = get(e.left, ch[2 * orig]);
         const double b = get(e.right, ch[2 * orig + 1]);
@@ -115,7 +132,12 @@ int main(int argc, char **argv) {
           printf("FAIL out slot\n");
           return 1;
         }
-        hbm[e.out] = x;
+        if (ch[2 * orig] < L && ch[2 * orig + 1] < L && !(e.meta & META_KEEP)) {
+          hbm[e.out] = NAN;  // not stored
+          crec[e.out] = Rec{ch[2 * orig], ch[2 * orig + 1], ints[orig]};
+        } else {
+          hbm[e.out] = x;
+        }
         const uint32_t push = e.meta & 0xFFu;
         if (push != NO_PUSH) {
           if (push >= s.capacity) { printf("FAIL push beyond capacity\n"); return 1; }
