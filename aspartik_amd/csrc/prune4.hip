@@ -106,10 +106,12 @@ __device__ __forceinline__ Mat load_mat(const float4 *p) {
   return m;
 }
 
-// x <- x / 2^e with e the binary exponent of max_j x_j (exact); returns e
+// x <- x / 2^e with e the binary exponent of max_j x_j (exact); returns e.
+// When all four values are 0 the site's likelihood is 0 whatever e is (its
+// ln L is -inf), so zero needs no separate case.
 __device__ __forceinline__ int normalise(float4 &v) {
   const float mx = fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w));
-  const int ex = (mx > 0.0f) ? __builtin_amdgcn_frexp_expf(mx) : 0;
+  const int ex = __builtin_amdgcn_frexp_expf(mx);
   v.x = __builtin_amdgcn_ldexpf(v.x, -ex);
   v.y = __builtin_amdgcn_ldexpf(v.y, -ex);
   v.z = __builtin_amdgcn_ldexpf(v.z, -ex);
@@ -206,6 +208,13 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
     p[q] = tile0 + t * 64u * PPL + q * 64u + lane;
     active[q] = p[q] < a.P;
     pp[q] = active[q] ? p[q] : 0u;
+  }
+  // byte offsets of this lane's patterns in a partial row and an exponent row
+  uint32_t poff[PPL], eoff[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    poff[q] = p[q] * 16u;
+    eoff[q] = p[q] * 2u;
   }
   const uint32_t ns = a.nslots;
   const size_t KP = (size_t)K * a.P;
@@ -346,7 +355,8 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
     const unsigned char *cs = code_s + (size_t)buf * CODEC;
     for (uint32_t j = 0; j < cnt; j++) {
       const uint32_t i = base + j;
-      const uint4 e4 = ent_s[i];
+      // through the scalar cache: no vector registers, no readfirstlane
+      const uint4 e4 = ((ConstU4 *)a.entries)[i];
       const uint32_t out = uni(e4.x), lsrc = uni(e4.y), rsrc = uni(e4.z),
                      meta = uni(e4.w);
       float4 cl[PPL], cr[PPL];
@@ -416,6 +426,9 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
       child(lsrc, 0, cl, cel, lv, le);
       child(rsrc, 1, cr, cer, rv, re);
       const uint32_t push = meta & 0xFFu;
+      const size_t ob = (size_t)out * KP + (size_t)c * a.P;
+      float4 *const pout = a.partials + ob;
+      int16_t *const eout = a.exps + ob;
       // a cherry's partial is not stored when its matrices are recorded,
       // unless this pass reads it back from memory (META_KEEP)
       const bool store = !(a.cmats && (lsrc & SRC_TAG) == SRC_TIP &&
@@ -427,28 +440,26 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
         v.y = lv[q].y * rv[q].y;
         v.z = lv[q].z * rv[q].z;
         v.w = lv[q].w * rv[q].w;
-        const float mx = fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w));
-        const int ex = (mx > 0.0f) ? __builtin_amdgcn_frexp_expf(mx) : 0;
-        v.x = __builtin_amdgcn_ldexpf(v.x, -ex);
-        v.y = __builtin_amdgcn_ldexpf(v.y, -ex);
-        v.z = __builtin_amdgcn_ldexpf(v.z, -ex);
-        v.w = __builtin_amdgcn_ldexpf(v.w, -ex);
+        const int ex = normalise(v);
         x[q] = v;
         E[q] = exp_clamp(le[q] + re[q] + ex);
+        const int16_t ep = exp_pack(E[q]);
         if (active[q] && store) {
-          const size_t o = (size_t)out * KP + (size_t)c * a.P + p[q];
+          // a uniform base plus the lane's loop-invariant 32-bit byte offset
+          f4v *dp = reinterpret_cast<f4v *>(reinterpret_cast<char *>(pout) + poff[q]);
+          int16_t *de = reinterpret_cast<int16_t *>(reinterpret_cast<char *>(eout) + eoff[q]);
 #if B3H_NT_STORE
           const f4v xv = {v.x, v.y, v.z, v.w};
-          __builtin_nontemporal_store(xv, reinterpret_cast<f4v *>(&a.partials[o]));
-          __builtin_nontemporal_store(exp_pack(E[q]), &a.exps[o]);
+          __builtin_nontemporal_store(xv, dp);
+          __builtin_nontemporal_store(ep, de);
 #else
-          a.partials[o] = v;
-          a.exps[o] = exp_pack(E[q]);
+          *dp = f4v{v.x, v.y, v.z, v.w};
+          *de = ep;
 #endif
         }
         if (push != NO_PUSH) {
           stk[(push * PPL + q) * 64 + lane] = v;
-          stke[(push * PPL + q) * 64 + lane] = exp_pack(E[q]);
+          stke[(push * PPL + q) * 64 + lane] = ep;
         }
       }
     }
@@ -595,16 +606,19 @@ __global__ __launch_bounds__(512) void prune_f32_batch_kernel(const F32Args *ins
 // kernel is compiled.  A wavefront issues the loads of kPathSeg sources at
 // once, runs the entries that read them from registers (one wait for the
 // whole segment instead of one memory latency per node), and moves on to
-// the next segment; eight waves per SIMD hide the rest (at 200 taxa and
-// 200,000 patterns, segments of 2-4 sources measured 58-59 us, 8 sources
-// 70 us, 16 sources 66 us).  Entries and matrices are the same for every
+// the next segment; eight waves per SIMD hide the rest.  At 128 taxa and
+// 100,000 patterns two sources per segment measured 36 us against 42 us for
+// four and eight (at 200 taxa and 200,000 patterns 2-4 sources measured
+// 58-59 us, 8 sources 70 us), and issuing the next segment's loads before
+// computing the current one was slower at every length (42-47 us): more
+// loads in flight per wave do not help here.  Entries and matrices are the same for every
 // lane of a wavefront and come through the scalar cache; LDS holds only the
 // category mix.  One lane per (pattern, category): wavefront (c, t) of a
 // workgroup of K x T wavefronts is category c of the t-th 64 patterns of
 // the workgroup's tile.  Arithmetic per node, root, mix and sum are those
 // of prune_tile -- products and exponent sums commute, so which side
 // carries the chain changes no bit -- and values match the other variants.
-constexpr uint32_t kPathSeg = 4;
+constexpr uint32_t kPathSeg = 2;
 
 typedef const B3H_CONSTANT F32Args ConstF32Args;
 
@@ -702,12 +716,7 @@ __global__ __launch_bounds__(1024) void prune_path_kernel(F32Args arg, const uin
         v.y = cv.y * sv4.y;
         v.z = cv.z * sv4.z;
         v.w = cv.w * sv4.w;
-        const float mx = fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w));
-        const int ex = (mx > 0.0f) ? __builtin_amdgcn_frexp_expf(mx) : 0;
-        v.x = __builtin_amdgcn_ldexpf(v.x, -ex);
-        v.y = __builtin_amdgcn_ldexpf(v.y, -ex);
-        v.z = __builtin_amdgcn_ldexpf(v.z, -ex);
-        v.w = __builtin_amdgcn_ldexpf(v.w, -ex);
+        const int ex = normalise(v);
         x = v;
         E = exp_clamp(ce + se + ex);
         // entry 0 may be a cherry: its matrices are recorded, its partial not
