@@ -94,6 +94,7 @@ class ChainConfig(C.Structure):
         ("tune_magnitude", C.c_double),
         ("full_evaluations", C.c_uint32), ("screen_topology", C.c_uint32),
         ("surrogate_beta", C.c_double),
+        ("narrow_reversible", C.c_uint32),
     ]
 
 
@@ -174,6 +175,9 @@ SIGNATURES = {
 # every symbol declared in include/b3mcmc.h
 MCMC_SIGNATURES = {
     "b3m_ess": (C.c_double, [f64p, C.c_size_t]),
+    "b3m_da_stage1": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                C.c_double]),
+    "b3m_da_stage2": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
     "b3m_mlp_create": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint64,
                                  C.POINTER(C.c_void_p)]),
     "b3m_mlp_destroy": (C.c_int, [C.c_void_p]),

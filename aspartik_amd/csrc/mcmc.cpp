@@ -176,10 +176,16 @@ struct Tree {
     replace_child(pa, a, b);
     replace_child(pb, b, a);
   }
-  bool is_grandparent(uint32_t v) const { return child(v, 0) >= L && child(v, 1) >= L; }
-  uint32_t num_grandparents() const {
+  // tree.rs:767-778 (two internal children); with `loose`, BEAST2's narrow
+  // Exchange (at least one internal child).  Leaves are never grandparents.
+  bool is_grandparent(uint32_t v, bool loose = false) const {
+    if (v < L) return false;
+    return loose ? (child(v, 0) >= L || child(v, 1) >= L)
+                 : (child(v, 0) >= L && child(v, 1) >= L);
+  }
+  uint32_t num_grandparents(bool loose = false) const {
     uint32_t n = 0;
-    for (uint32_t v = L; v < N; v++) n += is_grandparent(v);
+    for (uint32_t v = L; v < N; v++) n += is_grandparent(v, loose);
     return n;
   }
   double tree_length() const {
@@ -351,11 +357,14 @@ double propose(b3m_chain *c, int op) {
       return -std::log(s);
     }
     case kNarrow: {  // _tree_exchange.py:23-57
-      const uint32_t before = t.num_grandparents();
+      // b3's rule, or the reversible one (the grandparent keeps an internal
+      // child, `parent`, so its own status never changes)
+      const bool loose = c->cfg.narrow_reversible != 0;
+      const uint32_t before = t.num_grandparents(loose);
       if (before == 0) return -INFINITY;
       uint32_t gp;
       do gp = rng.range(t.L, t.N);
-      while (!t.is_grandparent(gp));
+      while (!t.is_grandparent(gp, loose));
       const uint32_t l = t.child(gp, 0), r = t.child(gp, 1);
       uint32_t parent, uncle;
       if (t.heights[l] > t.heights[r]) {
@@ -367,10 +376,15 @@ double propose(b3m_chain *c, int op) {
       } else {
         return -INFINITY;
       }
-      const uint32_t pre = (uint32_t)t.is_grandparent(parent) + (uint32_t)t.is_grandparent(uncle);
+      // b3's rule makes both children internal; under the loose one a dated
+      // leaf can be the higher child, and BEAST2 aborts then
+      if (parent < t.L) return -INFINITY;
+      const uint32_t pre =
+          (uint32_t)t.is_grandparent(parent, loose) + (uint32_t)t.is_grandparent(uncle, loose);
       const uint32_t ch = rng.uniform() < 0.5 ? t.child(parent, 0) : t.child(parent, 1);
       t.swap_parents(uncle, ch);
-      const uint32_t post = (uint32_t)t.is_grandparent(parent) + (uint32_t)t.is_grandparent(uncle);
+      const uint32_t post =
+          (uint32_t)t.is_grandparent(parent, loose) + (uint32_t)t.is_grandparent(uncle, loose);
       const uint32_t after = before - pre + post;
       return std::log((double)before / (double)after);
     }
@@ -513,6 +527,7 @@ void b3m_chain_config_default(b3m_chain_config *cfg) {
   cfg->tune_cutoff = 0;
   cfg->tune_magnitude = 0.5;
   cfg->surrogate_beta = 1.0;
+  cfg->narrow_reversible = 0;
 }
 
 b3h_status b3m_chain_create(const b3m_chain_config *cfg, const uint8_t *codes, uint32_t L,
@@ -805,9 +820,9 @@ b3h_status b3m_chain_run(b3m_chain *c, uint64_t steps, uint32_t mode, uint32_t r
           finalize(c, false);  // an error ends the run with the proposal rejected
           break;
         }
-        const double df = c->cfg.surrogate_beta * (fnew - c->surrogate_cur);
-        const double r1 = df + (prior - c->logprior) + hastings;
-        if (!(r1 > std::log(c->rng.uniform()))) {
+        const double beta = c->cfg.surrogate_beta;
+        if (!b3m_da_stage1(beta, c->surrogate_cur, fnew, prior - c->logprior, hastings,
+                           c->rng.uniform())) {
           s.stage1_rejects++;
         } else {
           double ll = 0.0;
@@ -815,7 +830,9 @@ b3h_status b3m_chain_run(b3m_chain *c, uint64_t steps, uint32_t mode, uint32_t r
             finalize(c, false);
             break;
           }
-          if (ll != -INFINITY && (ll - c->loglik) - df > std::log(c->rng.uniform())) {
+          // the uniform is drawn only for a finite ln L, as it always was
+          if (ll != -INFINITY &&
+              b3m_da_stage2(beta, c->surrogate_cur, fnew, ll - c->loglik, c->rng.uniform())) {
             accept = true;
             c->loglik = ll;
             c->logprior = prior;
@@ -881,6 +898,17 @@ b3h_status b3m_chain_run(b3m_chain *c, uint64_t steps, uint32_t mode, uint32_t r
   s.tuning[kAlpha] = c->alpha_f;
   if (stats) *stats = s;
   return st;
+}
+
+int b3m_da_stage1(double beta, double f_cur, double f_new, double d_prior, double hastings,
+                  double u1) {
+  const double r1 = beta * (f_new - f_cur) + d_prior + hastings;
+  return r1 > std::log(u1) ? 1 : 0;
+}
+
+int b3m_da_stage2(double beta, double f_cur, double f_new, double d_loglik, double u2) {
+  if (d_loglik == -INFINITY || std::isnan(d_loglik)) return 0;
+  return d_loglik - beta * (f_new - f_cur) > std::log(u2) ? 1 : 0;
 }
 
 }  // extern "C"

@@ -107,10 +107,11 @@ def bench_c1(args):
     patterns, 10,000 plain MCMC steps with every node recomputed at each
     likelihood call -- timed end to end with the native chain driver
     (NodeSlide, TreeScale, NarrowExchange at b3's weights; the substitution
-    model fixed) and its GPU likelihood.  cpu_baseline: the oracle's FP64
-    likelihood (cpu4.rs threading: one thread below 1,000 patterns) over the
-    same number of full evaluations, the likelihood work of such a run.  At
-    this size a step is launch- and synchronisation-bound on the GPU."""
+    model fixed) and its GPU likelihood.  cpu_baseline: the same chain on the
+    CPU in FP64 (oracle/chain.py: b3's step and operators over the oracle's
+    tree and likelihood, one thread as cpu4.rs picks below 1,000 patterns),
+    timed end to end over a bounded number of steps.  At this size a step is
+    launch- and synchronisation-bound on the GPU."""
     from aspartik_amd import workloads
     from aspartik_amd.b3 import Chain
 
@@ -131,29 +132,32 @@ def bench_c1(args):
     log(f"[bench] C1 chain: {dt:.2f} s for {steps} steps; accepted {st['accepted']}")
     cpu, err = None, None
     if not args.no_cpu:
+        from oracle import chain as OC
         from oracle import oracle as O
         ot = O.Tree(L, par, ch.astype(np.uint32), h)
         om = O.Model.hky(wl["kappa"], wl["freqs"])
         lik = O.GammaLikelihood(ot, om, wl["codes"], P, wl["weights"], num_categories=1,
                                 threads_per_category=1)
         ref = lik.likelihood()
-        lik.accept()
-        ot.accept()
         err = {"final_state_rel": abs(loglik - ref) / abs(ref), "ref": ref, "gpu": loglik}
-        nev = 0
-        t0 = time.perf_counter()
-        while nev < steps and time.perf_counter() - t0 < args.cpu_seconds:
-            ot.mark_all_edges_updated()
-            lik.likelihood()
-            lik.accept()
-            ot.accept()
-            nev += 1
-        dtc = time.perf_counter() - t0
-        cpu = {"value": dtc / nev * steps, "unit": "s per 10,000 steps (likelihood work)",
-               "cores": 1, "kind": "port",
-               "sample": f"{nev} full FP64 evaluations of the C1 workload ({dtc:.2f} s), "
-                         f"scaled to {steps}"}
-        log(f"[bench] C1 oracle: {cpu['value']:.2f} s per {steps} evaluations; final-state "
+        # the same chain on the CPU in FP64: b3's step and the three operators
+        # over the oracle's tree and likelihood, every node recomputed per step,
+        # from the same starting state (one thread, as cpu4.rs picks below
+        # 1,000 patterns)
+        oc = OC.Chain(L, wl["parents"], wl["children"], wl["heights"], om, wl["codes"],
+                      wl["weights"], birth_rate=1.0, seed=4)
+        oc.run(200)
+        budget = int(min(steps, max(1000, steps * args.cpu_seconds / 30.0)))
+        cst = oc.run(budget)
+        dtc = cst["seconds"]
+        cpu = {"value": dtc / budget * 10_000, "unit": "s per 10,000 steps", "cores": 1,
+               "kind": "port", "accepted_fraction": cst["accepted"] / cst["steps"],
+               "sample": f"{budget} steps of the same chain (NodeSlide, TreeScale, "
+                         f"NarrowExchange at 4:1:2; Yule prior; FP64 full evaluations) on the "
+                         f"CPU ({dtc:.2f} s), scaled to 10,000; GPU chain accepted "
+                         f"{st['accepted'] / steps:.3f} of its steps, CPU chain "
+                         f"{cst['accepted'] / cst['steps']:.3f}"}
+        log(f"[bench] C1 CPU chain: {cpu['value']:.2f} s per 10,000 steps; final-state "
             f"ln L rel err {err['final_state_rel']:.1e}")
     print(json.dumps({
         "metric": "seconds per 10,000 plain MCMC steps (16 taxa, HKY, 1,000 sites)",
@@ -368,6 +372,21 @@ def bench_codon(args):
     }), flush=True)
 
 
+def ess_batch_means(x):
+    """Batch-means effective sample size, a cross-check of Geyer's estimate:
+    n s^2 / (b s_b^2) with b = floor(sqrt(n)) samples per batch and s_b^2
+    the variance of the batch means."""
+    x = np.asarray(x, np.float64)
+    n = x.size
+    b = max(1, int(np.sqrt(n)))
+    m = n // b
+    if m < 2:
+        return float("nan")
+    means = x[: m * b].reshape(m, b).mean(1)
+    var_bm = b * means.var(ddof=1)
+    return float(n * x.var(ddof=1) / var_bm) if var_bm > 0 else float("nan")
+
+
 def _run_chunked(chain, steps, mode, name, chunk=50_000, max_rows=100_000):
     """Run a chain in chunks of ``chunk`` steps on the same handle (the chain
     state carries over, so the steps are those of one run), logging each
@@ -401,13 +420,16 @@ def bench_mcmc(args):
     200 taxa, 200,000 sites.  Burn-in with plain MH collects (features,
     exact ln L) pairs; the MLP surrogate (404 -> 256 -> 256 -> 1) is trained
     on them; then plain and delayed-acceptance chains run the same number of
-    steps from the burn-in's final state (with --screen-topology 1, the
-    default, NarrowExchange passes the surrogate's first stage too; one
-    delayed chain per first-stage temperature in --betas, the first being
-    the headline).  ESS is Geyer's estimator on the log posterior, tree
-    length and alpha traces (recorded every ceil(steps / 100,000) steps);
-    ESS/s uses each chain's wall clock (likelihood, surrogate and host
-    work)."""
+    steps from the burn-in's final state (topology moves go straight to the
+    exact test; with --screen-topology 1 NarrowExchange passes the
+    surrogate's first stage too; one delayed chain per first-stage
+    temperature in --betas, the first being the headline).  NarrowExchange
+    takes its reversible form by default (--narrow-reversible), so both
+    chains keep the same posterior and their summaries can be compared.
+    ESS is Geyer's estimator on the log posterior, tree length and alpha
+    traces (recorded every ceil(steps / 100,000) steps), cross-checked by
+    batch means; ESS/s uses each chain's wall clock (likelihood, surrogate
+    and host work)."""
     from aspartik_amd import workloads
     from aspartik_amd.b3 import Chain, Surrogate, ess
 
@@ -415,7 +437,8 @@ def bench_mcmc(args):
     wl = workloads.gtr_gamma4_mcmc(args.mcmc_taxa, args.mcmc_sites)
     L, P = args.mcmc_taxa, args.mcmc_sites
     kw = dict(birth_rate=float((L - 1) / (wl["heights"][wl["parents"][:-1].astype(np.int64)]
-                                          - wl["heights"][:-1]).sum()))
+                                          - wl["heights"][:-1]).sum()),
+              narrow_reversible=args.narrow_reversible)
     # operator tuning during burn-in (b3's LinearOptimizer schedule), frozen after
     burn = Chain(wl["codes"], wl["parents"], wl["children"], wl["heights"], wl["gtr_rates"],
                  wl["freqs"], wl["alpha"], seed=101, tune_every=max(50, args.burnin // 40),
@@ -468,14 +491,24 @@ def bench_mcmc(args):
             c.set_surrogate(sur)
         trace, st = _run_chunked(c, args.mcmc_steps, mode, name)
         c.close()
-        e = {k: ess(trace[:, col]) for k, col in (("posterior", 1), ("tree_length", 5),
-                                                   ("alpha", 6))}
-        res[name] = dict(stats=st, ess=e, min_ess=min(e.values()),
-                         ess_per_s=min(e.values()) / st["seconds"])
+        cols = (("posterior", 1), ("tree_length", 5), ("alpha", 6))
+        e = {k: ess(trace[:, col]) for k, col in cols}
+        ebm = {k: ess_batch_means(trace[:, col]) for k, col in cols}
+        res[name] = dict(stats=st, ess=e, ess_batch_means=ebm, min_ess=min(e.values()),
+                         ess_per_s=min(e.values()) / st["seconds"],
+                         ess_per_s_by_statistic={k: v / st["seconds"] for k, v in e.items()},
+                         means={k: float(trace[:, col].mean()) for k, col in cols})
         log(f"[bench] {name}: {st['seconds']:.1f} s (likelihood {st['likelihood_seconds']:.1f} s, "
             f"surrogate {st['surrogate_seconds']:.1f} s), accepted {st['op_accepted']} of "
             f"{st['op_proposed']}, exact {st['exact_evals']}, ESS {e}")
     ratio = res["delayed"]["ess_per_s"] / res["plain"]["ess_per_s"]
+    ratio_by_stat = {k: res["delayed"]["ess_per_s_by_statistic"][k] /
+                     res["plain"]["ess_per_s_by_statistic"][k]
+                     for k in res["plain"]["ess_per_s_by_statistic"]}
+    ratio_bm = {k: (res["delayed"]["ess_batch_means"][k] / res["delayed"]["stats"]["seconds"]) /
+                (res["plain"]["ess_batch_means"][k] / res["plain"]["stats"]["seconds"])
+                for k in res["plain"]["ess_batch_means"]}
+    log(f"[bench] ESS/s ratios (Geyer) {ratio_by_stat}; batch means {ratio_bm}")
     ratio_incl_training = (res["delayed"]["min_ess"] /
                            (res["delayed"]["stats"]["seconds"] + t_train)) / res["plain"]["ess_per_s"]
     cpu = None
@@ -522,9 +555,14 @@ def bench_mcmc(args):
                    "topology_moves": "surrogate stage, then exact" if args.screen_topology
                                      else "exact MH test only",
                    "operators": "NodeSlide 4, TreeScale 1, DeltaExchange 2, ScaleReal(alpha) 1, "
-                                "NarrowExchange 2 (b3's defaults, tuned in burn-in)"},
+                                "NarrowExchange 2 (b3's defaults, tuned in burn-in)",
+                   "narrow_exchange": "reversible (grandparent: an internal child)"
+                                      if args.narrow_reversible
+                                      else "b3's rule (grandparent: two internal children)"},
         "ess_per_s_plain": res["plain"]["ess_per_s"], "ess_per_s_delayed": res["delayed"]["ess_per_s"],
         "ess_ratio_delayed_vs_plain": ratio,
+        "ess_ratio_by_statistic": ratio_by_stat,
+        "ess_ratio_by_statistic_batch_means": ratio_bm,
         "ess_ratio_including_training": ratio_incl_training,
         "plain": res["plain"], "delayed": res["delayed"], "surrogate_held_out": held,
         "other_betas": {n: dict(v, ess_ratio_vs_plain=v["ess_per_s"] / res["plain"]["ess_per_s"])
@@ -794,15 +832,19 @@ def main():
     ap.add_argument("--burnin", type=int, default=60_000)
     ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--clip", type=float, default=100.0)
-    ap.add_argument("--collect", type=int, default=30_000,
+    ap.add_argument("--collect", type=int, default=50_000,
                     help="surrogate training pairs from the last N burn-in steps (0: all)")
     ap.add_argument("--betas", default="0.5",
                     help="delayed acceptance: first-stage temperatures, one chain each; "
                          "the first is the headline")
     ap.add_argument("--chain-seed", type=int, default=202,
                     help="seed of the plain chain; delayed chains take the next ones")
-    ap.add_argument("--screen-topology", type=int, choices=[0, 1], default=1,
+    ap.add_argument("--screen-topology", type=int, choices=[0, 1], default=0,
                     help="delayed acceptance: topology moves pass the surrogate stage too")
+    ap.add_argument("--narrow-reversible", type=int, choices=[0, 1], default=1,
+                    help="ESS workload: NarrowExchange in its reversible form (1), which "
+                         "keeps the posterior, or b3's rule (0), which has no reverse move "
+                         "when the exchanged child is a leaf")
     ap.add_argument("--codon-taxa", type=int, default=64)
     ap.add_argument("--codon-patterns", type=int, default=20_000)
     ap.add_argument("--instances", type=int, default=256)

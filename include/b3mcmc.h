@@ -36,6 +36,22 @@ extern "C" {
  * for n < 4 or a constant trace. */
 double b3m_ess(const double *x, size_t n);
 
+/* ---- the delayed-acceptance rule -------------------------------------- */
+
+/* The two tests of one delayed-acceptance MH step (Christen & Fox 2005), as
+ * the chain applies them; host only, no reference counterpart (b3 has no
+ * surrogate).  f is the surrogate's ln L, tempered by beta in the first
+ * stage: the first stage accepts iff
+ *     beta (f_new - f_cur) + d_prior + hastings > ln u1,
+ * i.e. with probability min(1, e^{beta df + d_prior + h}); the exact stage
+ * then accepts iff
+ *     d_loglik - beta (f_new - f_cur) > ln u2
+ * (d_loglik = -inf rejects).  Together the chain targets the exact
+ * posterior.  u1, u2 in (0, 1). */
+int b3m_da_stage1(double beta, double f_cur, double f_new, double d_prior, double hastings,
+                  double u1);
+int b3m_da_stage2(double beta, double f_cur, double f_new, double d_loglik, double u2);
+
 /* ---- MLP surrogate ---------------------------------------------------- */
 
 /* inputs -> hidden -> hidden -> 1, GELU (tanh form) after both hidden
@@ -109,6 +125,16 @@ typedef struct {
    * beta < 1 lets more moves reach the exact test when f is noisy; any
    * beta > 0 keeps the exact posterior (default 1) */
   double surrogate_beta;
+  /* NarrowExchange's notion of a grandparent.  0 (default): b3's, an
+   * internal node with two internal children (tree.rs:767-770,
+   * _tree_exchange.py:23-57).  Exchanging a leaf child of `parent` then
+   * leaves the grandparent with a leaf child, so the reverse move cannot be
+   * proposed and the chain does not keep its posterior (on 4 taxa a
+   * balanced tree can become a caterpillar but never return).  1: BEAST2's
+   * narrow Exchange, an internal node with at least one internal child; the
+   * grandparent keeps that status, every move has its reverse, and the
+   * Hastings term ln(V_before / V_after) counts such nodes. */
+  uint32_t narrow_reversible;
 } b3m_chain_config;
 
 void b3m_chain_config_default(b3m_chain_config *cfg);

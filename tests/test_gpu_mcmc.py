@@ -198,6 +198,37 @@ def test_chain_loglik_matches_oracle_after_many_moves(hip_device):
     assert all(n > 0 for n in stats["op_proposed"])
 
 
+def test_narrow_exchange_topology_frequencies_with_no_data(hip_device):
+    """NarrowExchange on 4 taxa with every tip unknown (ln L = 0): the prior
+    gives every labelled ranked history the same mass (the tree length is
+    the sum of the internal heights plus the root's whatever the topology),
+    so a balanced tree has probability 6 / 18 = 1/3.  narrow_reversible=1
+    (a grandparent has at least one internal child) gets it within 5
+    standard errors; b3's rule (two internal children, tree.rs:767-770)
+    turns the balanced start into a caterpillar that never returns, as the
+    CPU chain of the oracle does."""
+    parents = np.array([4, 4, 5, 5, 6, 6, 0xFFFFFFFF], np.uint32)
+    children = np.array([[0, 1], [2, 3], [4, 5]], np.uint32)
+    heights = np.array([0.0, 0.0, 0.0, 0.0, 0.2, 0.3, 0.5])
+    codes = np.full((4, 2), 15, np.uint8)
+    out = {}
+    for loose in (0, 1):
+        c = Chain(codes, parents, children, heights, np.ones(6), np.full(4, 0.25), 1.0,
+                  categories=1, w_rates=0.0, w_alpha=0.0, birth_rate=2.0, seed=7,
+                  narrow_reversible=loose)
+        flags = []
+        for _ in range(12_000):
+            c.run(1)
+            par, ch, _, _, _ = c.params()
+            root = int(np.flatnonzero(par == 0xFFFFFFFF)[0])
+            flags.append(float(all(int(v) >= 4 for v in ch[root - 4])))
+        c.close()
+        out[loose] = np.array(flags[500:])
+    assert out[0][-5000:].sum() == 0.0
+    m, se = _mean_and_se(out[1])
+    assert abs(m - 1.0 / 3.0) < 5 * se, (m, se)
+
+
 def _mean_and_se(x):
     e = max(ess(x), 2.0)
     return x.mean(), x.std() / math.sqrt(e)
@@ -243,3 +274,45 @@ def test_delayed_acceptance_keeps_the_posterior(hip_device):
             m1, s1 = _mean_and_se(runs[other][0][:, col])
             z = abs(m0 - m1) / math.hypot(s0, s1)
             assert z < 5.0, (col_name, other, m0, m1, s0, s1)
+
+
+def test_trained_surrogate_gains_over_untrained_and_plain(hip_device):
+    """What the trained network gains.  Delayed-acceptance chains at the same
+    seeds, every move screened (so every exact evaluation follows a passed
+    first stage), one with the trained surrogate and one with an untrained
+    network of the same shape and input scaling; plain MH beside them.  The
+    trained first stage must pass moves the exact test then accepts far more
+    often (stage-2 acceptance), so each accepted move costs fewer exact
+    evaluations than with the untrained network and than plain MH.  b3 has
+    no counterpart (it has no surrogate)."""
+    d = _data(8, 400, 3)
+    kw = dict(birth_rate=2.0, w_narrow=1.0)
+    burn = Chain(d["codes"], d["parents"], d["children"], d["heights"], d["rates"], d["freqs"],
+                 d["alpha"], seed=7, **kw)
+    _, _, (X, y) = burn.run(4000, collect=4000)
+    par, ch, h, r, a = burn.params()
+    nf = burn.num_features
+    burn.close()
+    trained = Surrogate(nf, 128, seed=1)
+    trained.fit_scaling(X, y)
+    trained.train(X, y, epochs=200, batch=256, lr=1e-3)
+    untrained = Surrogate(nf, 128, seed=2)
+    untrained.fit_scaling(X, y)
+    out = {}
+    for name, sur in (("trained", trained), ("untrained", untrained), ("plain", None)):
+        c = Chain(d["codes"], par, ch, h, r, d["freqs"], a, seed=31, screen_topology=1, **kw)
+        if sur is not None:
+            c.set_surrogate(sur)
+        _, st, _ = c.run(10000, mode="plain" if sur is None else "delayed")
+        c.close()
+        out[name] = st
+    t, u, p = out["trained"], out["untrained"], out["plain"]
+    stage2 = {k: out[k]["accepted"] / out[k]["exact_evals"] for k in ("trained", "untrained")}
+    per_accept = {k: out[k]["exact_evals"] / out[k]["accepted"] for k in out}
+    print(f"[gain] stage-2 acceptance {stage2}; exact evaluations per accepted move {per_accept}",
+          flush=True)
+    assert t["accepted"] > 100 and u["accepted"] > 20 and p["accepted"] > 100
+    assert stage2["trained"] > 1.5 * stage2["untrained"], stage2
+    assert per_accept["trained"] < 0.7 * per_accept["untrained"], per_accept
+    assert per_accept["trained"] < 0.7 * per_accept["plain"], per_accept
+    assert t["exact_evals"] < 0.7 * p["exact_evals"], (t["exact_evals"], p["exact_evals"])

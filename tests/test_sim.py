@@ -23,6 +23,57 @@ def test_yule_tree_structure():
         assert (lengths > 0).all()
 
 
+def _spacings(L, rng, birth_rate=1.0):
+    """k * (waiting time while k lineages remain), k = L .. 2, of one tree:
+    iid Exp(birth_rate) before the rescaling to the mean branch length, so
+    iid Exp(birth_rate / c) after it, for the tree's own factor c."""
+    par, ch, h = sim.yule_tree(L, rng, birth_rate, 0.1)
+    t = np.sort(h[L:])                      # internal heights, leaves at 0
+    dt = np.diff(np.concatenate([[0.0], t]))
+    k = np.arange(L, 1, -1)                 # lineages during each interval
+    return k * dt, ch
+
+
+def test_yule_waiting_times_are_exponential():
+    """Yule waiting times Exp(k lambda) (the density priors/yule.rs:23-40
+    is the product of): for two intervals a != b of one tree, y_a / (y_a +
+    y_b) is Uniform(0, 1) whatever the tree's rescaling factor.  KS tests
+    over independent trees, one pair per tree, at two pairs of intervals; a
+    generator that drew Exp(lambda) instead of Exp(k lambda) fails them."""
+    from scipy import stats
+    rng = np.random.default_rng(17)
+    L = 30
+    u_first_last, u_mid = [], []
+    for _ in range(1500):
+        y, _ = _spacings(L, rng, birth_rate=2.0)
+        u_first_last.append(y[0] / (y[0] + y[-1]))   # k = L and k = 2
+        u_mid.append(y[10] / (y[10] + y[20]))
+    for u in (u_first_last, u_mid):
+        assert stats.kstest(u, "uniform").pvalue > 1e-3
+    # the check has power: the same ratio of the raw waiting times, which a
+    # generator drawing Exp(lambda) for every k would make uniform, is far
+    # from uniform for this one (k = L intervals are L / 2 times shorter)
+    bad = []
+    for _ in range(1500):
+        y, _ = _spacings(L, rng)
+        dt = y / np.arange(L, 1, -1)
+        bad.append(dt[0] / (dt[0] + dt[-1]))
+    assert stats.kstest(bad, "uniform").pvalue < 1e-6
+
+
+def test_yule_merges_uniform_pairs():
+    """Uniform choice of the merging pair gives E[#cherries] = L / 3 for a
+    Yule tree on L >= 3 leaves (McKenzie & Steel 2000)."""
+    rng = np.random.default_rng(5)
+    L, n = 40, 3000
+    cherries = []
+    for _ in range(n):
+        _, ch = _spacings(L, rng)
+        cherries.append(int(((ch[:, 0] < L) & (ch[:, 1] < L)).sum()))
+    cherries = np.array(cherries, float)
+    assert abs(cherries.mean() - L / 3) < 5 * cherries.std() / np.sqrt(n)
+
+
 def test_reversible_transitions_match_expm():
     m = HKY([0.1, 0.2, 0.3, 0.4], 3.0)
     q = m.q_matrix()
