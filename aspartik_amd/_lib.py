@@ -127,6 +127,17 @@ f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 
 _lib = None
 
+def addr(a) -> int:
+    """Address of the first element of a C-contiguous array, for the
+    pointer arguments of the per-evaluation calls (declared c_void_p: numpy's
+    ndpointer check costs microseconds per argument).  The caller converts
+    dtype and layout and keeps the array alive across the call."""
+    try:
+        return C.addressof(C.c_char.from_buffer(a))
+    except (TypeError, ValueError):  # read-only or empty
+        return a.ctypes.data
+
+
 # name -> (restype, argtypes); every symbol declared in include/b3hip.h
 SIGNATURES = {
     "b3h_config_default": (None, [C.POINTER(Config)]),
@@ -134,15 +145,15 @@ SIGNATURES = {
                              C.POINTER(C.c_void_p)]),
     "b3h_destroy": (C.c_int, [C.c_void_p]),
     "b3h_set_weights": (C.c_int, [C.c_void_p, u32p]),
-    "b3h_propose": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, f64p, f64p]),
+    "b3h_propose": (C.c_int, [C.c_void_p, C.c_uint32] + [C.c_void_p] * 4),
     "b3h_set_model": (C.c_int, [C.c_void_p, f64p, f64p, C.c_uint32]),
-    "b3h_propose_times": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, f64p, f64p]),
-    "b3h_propose_propagations": (C.c_int, [C.c_void_p, C.c_uint32, u32p, C.c_uint32, u32p,
-                                           f64p, f64p]),
-    "b3h_propose_propagations_times": (C.c_int, [C.c_void_p, C.c_uint32, u32p, C.c_uint32, u32p,
-                                                 f64p, f64p]),
+    "b3h_propose_times": (C.c_int, [C.c_void_p, C.c_uint32] + [C.c_void_p] * 4),
+    "b3h_propose_propagations": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]
+                                 + [C.c_void_p] * 3),
+    "b3h_propose_propagations_times": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p,
+                                                 C.c_uint32] + [C.c_void_p] * 3),
     "b3h_set_category_model": (C.c_int, [C.c_void_p, C.c_uint32, f64p, f64p, C.c_uint32]),
-    "b3h_propose_lengths": (C.c_int, [C.c_void_p, C.c_uint32, u32p, u32p, f64p]),
+    "b3h_propose_lengths": (C.c_int, [C.c_void_p, C.c_uint32] + [C.c_void_p] * 3),
     "b3h_set_classes": (C.c_int, [C.c_void_p, C.c_void_p]),
     "b3h_accept": (C.c_int, [C.c_void_p]),
     "b3h_reject": (C.c_int, [C.c_void_p]),

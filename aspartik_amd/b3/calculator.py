@@ -12,7 +12,7 @@ import ctypes as C
 import numpy as np
 
 from .. import _lib
-from .._lib import check, lib
+from .._lib import addr, check, lib
 
 
 class Calculator:
@@ -77,6 +77,13 @@ class Calculator:
                 f"scale_ln={self.scale_ln})")
 
 
+def _sizes(*ok):
+    """The arrays of a propose are passed as bare pointers: their sizes are
+    checked here."""
+    if not all(ok):
+        raise ValueError("array sizes do not match the update list")
+
+
 class HipCalculator:
     """One libb3hip handle: K rate categories of one alignment."""
 
@@ -87,6 +94,7 @@ class HipCalculator:
             raise ValueError("samples size is not a multiple of num_patterns")
         self.L = samples.size // self.P
         self.K = int(num_categories)
+        self.S = int(config.states)
         cfg = _lib.Config()
         lib().b3h_config_default(C.byref(cfg))
         cfg.device = config.device
@@ -113,38 +121,42 @@ class HipCalculator:
 
     # ---- Calculator<f64> -------------------------------------------------
     def propose(self, internals, children, matrices, frequencies):
-        internals = np.ascontiguousarray(internals, np.uint32)
-        check(lib().b3h_propose(self._h, internals.size, internals,
-                                np.ascontiguousarray(children, np.uint32).reshape(-1),
-                                np.ascontiguousarray(matrices, np.float64).reshape(-1),
-                                np.ascontiguousarray(frequencies, np.float64)))
+        nodes = np.ascontiguousarray(internals, np.uint32)
+        ch = np.ascontiguousarray(children, np.uint32)
+        m = np.ascontiguousarray(matrices, np.float64)
+        fr = np.ascontiguousarray(frequencies, np.float64)
+        _sizes(ch.size == 2 * nodes.size, m.size == self.K * 2 * nodes.size * self.S * self.S,
+               fr.size >= self.S)
+        check(lib().b3h_propose(self._h, nodes.size, addr(nodes), addr(ch), addr(m), addr(fr)))
 
     def propose_propagations(self, nodes, children, num_leaves, matrices, frequencies):
         """CudaPropagationsCalculator::propose: propagation lists and one
         matrix per listed node but the root, [K][len(nodes) - 1][4][4]."""
         nodes = np.ascontiguousarray(nodes, np.uint32)
-        check(lib().b3h_propose_propagations(
-            self._h, nodes.size, nodes, int(num_leaves),
-            np.ascontiguousarray(children, np.uint32).reshape(-1),
-            np.ascontiguousarray(matrices, np.float64).reshape(-1),
-            np.ascontiguousarray(frequencies, np.float64)))
+        ch = np.ascontiguousarray(children, np.uint32)
+        m = np.ascontiguousarray(matrices, np.float64)
+        fr = np.ascontiguousarray(frequencies, np.float64)
+        check(lib().b3h_propose_propagations(self._h, nodes.size, addr(nodes), int(num_leaves),
+                                             addr(ch), addr(m), addr(fr)))
 
     def propose_propagations_times(self, nodes, children, num_leaves, edge_times,
                                    category_rates):
         """edge_times: time length of each listed node's edge but the root's."""
         nodes = np.ascontiguousarray(nodes, np.uint32)
-        check(lib().b3h_propose_propagations_times(
-            self._h, nodes.size, nodes, int(num_leaves),
-            np.ascontiguousarray(children, np.uint32).reshape(-1),
-            np.ascontiguousarray(edge_times, np.float64),
-            np.ascontiguousarray(category_rates, np.float64)))
+        ch = np.ascontiguousarray(children, np.uint32)
+        t = np.ascontiguousarray(edge_times, np.float64)
+        r = np.ascontiguousarray(category_rates, np.float64)
+        check(lib().b3h_propose_propagations_times(self._h, nodes.size, addr(nodes),
+                                                   int(num_leaves), addr(ch), addr(t), addr(r)))
 
     def propose_times(self, internals, children, edge_times, category_rates):
-        internals = np.ascontiguousarray(internals, np.uint32)
-        check(lib().b3h_propose_times(self._h, internals.size, internals,
-                                      np.ascontiguousarray(children, np.uint32).reshape(-1),
-                                      np.ascontiguousarray(edge_times, np.float64),
-                                      np.ascontiguousarray(category_rates, np.float64)))
+        nodes = np.ascontiguousarray(internals, np.uint32)
+        ch = np.ascontiguousarray(children, np.uint32)
+        t = np.ascontiguousarray(edge_times, np.float64)
+        r = np.ascontiguousarray(category_rates, np.float64)
+        _sizes(ch.size == 2 * nodes.size, t.size == 2 * nodes.size, r.size == self.K)
+        check(lib().b3h_propose_times(self._h, nodes.size, addr(nodes), addr(ch), addr(t),
+                                      addr(r)))
 
     def set_model(self, q, frequencies, special_cases: bool = True):
         flags = _lib.B3H_MODEL_SPECIAL_CASES if special_cases else 0
@@ -160,13 +172,14 @@ class HipCalculator:
 
     def propose_lengths(self, internals, children, lengths):
         """lengths [K][2n]: branch length of each child edge per category."""
-        internals = np.ascontiguousarray(internals, np.uint32)
-        lengths = np.ascontiguousarray(lengths, np.float64).reshape(-1)
-        if lengths.size != self.K * 2 * internals.size:
+        nodes = np.ascontiguousarray(internals, np.uint32)
+        lengths = np.ascontiguousarray(lengths, np.float64)
+        if lengths.size != self.K * 2 * nodes.size:
             raise ValueError("lengths must hold K x 2n values")
-        check(lib().b3h_propose_lengths(self._h, internals.size, internals,
-                                        np.ascontiguousarray(children, np.uint32).reshape(-1),
-                                        lengths))
+        ch = np.ascontiguousarray(children, np.uint32)
+        _sizes(ch.size == 2 * nodes.size)
+        check(lib().b3h_propose_lengths(self._h, nodes.size, addr(nodes), addr(ch),
+                                        addr(lengths)))
 
     def set_classes(self, classes):
         """Per-pattern category (hetero.rs:150-160); None: category mean."""

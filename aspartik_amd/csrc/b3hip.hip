@@ -91,6 +91,11 @@ struct b3h_calc {
   uint32_t seq = 0, expect[2] = {0u, 0u};  // last number issued; expected per slot
   unsigned char *d_stage[kStages] = {nullptr, nullptr};
   hipEvent_t stage_ev[kStages] = {nullptr, nullptr};
+  // a stage whose last reader is a kernel that stores a sequence number in
+  // h_flag is free once either slot's flag has reached that number (the
+  // stream runs in order); the others are released by stage_ev
+  uint32_t stage_seq[kStages] = {0u, 0u};
+  bool stage_by_flag[kStages] = {false, false};
   int stage_next = 0;
 
   // host state
@@ -221,10 +226,32 @@ b3h_status validate_list(const b3h_calc *c, uint32_t n,
   return B3H_OK;
 }
 
+bool seq_reached(const b3h_calc *c, uint32_t q) {
+  for (int slot = 0; slot < 2; slot++)
+    if ((int32_t)(__atomic_load_n(c->h_flag + slot, __ATOMIC_ACQUIRE) - q) >= 0) return true;
+  return false;
+}
+
 unsigned char *next_stage(b3h_calc *c, int *which) {
   int s = c->stage_next;
   c->stage_next = (s + 1) % kStages;
-  (void)hipEventSynchronize(c->stage_ev[s]);  // previous copy from it is done
+  if (c->stage_by_flag[s]) {
+    // poll (up to 20 ms) for the last reader's sequence number, else wait
+    // for the stream, which a failed launch also ends
+    bool done = seq_reached(c, c->stage_seq[s]);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 1; !done; i++) {
+      __builtin_ia32_pause();
+      done = seq_reached(c, c->stage_seq[s]);
+      if (!done && (i & 1023u) == 0 &&
+          std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20))
+        break;
+    }
+    if (!done) (void)hipStreamSynchronize(c->stream);
+    c->stage_by_flag[s] = false;
+  } else {
+    (void)hipEventSynchronize(c->stage_ev[s]);  // previous copy from it is done
+  }
   *which = s;
   return c->h_stage[s];
 }
@@ -454,7 +481,6 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
                                 reinterpret_cast<const double *>(h + off_r), c->d_model, mstride,
                                 static_cast<float *>(c->d_pmats), c->stream, h, d,
                                 (uint32_t)((total + 15) / 16), c->d_cmats, c->d_cleaves));
-    HIPCHK(hipEventRecord(c->stage_ev[s], c->stream));  // the last reader of h
     F32Args a{};
     a.entries = d_entries;
     a.pmats = static_cast<const float *>(c->d_pmats);
@@ -471,6 +497,10 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     a.host_total = c->h_total + ws;
     a.host_flag = c->h_flag + ws;
     a.seq = c->expect[ws] = ++c->seq;
+    // the P(t) build, the last reader of h, runs before this kernel stores
+    // its sequence number
+    c->stage_seq[s] = a.seq;
+    c->stage_by_flag[s] = true;
     a.pi0 = freqs[0];
     a.pi1 = freqs[1];
     a.pi2 = freqs[2];

@@ -222,6 +222,6 @@ def test_hetero_error_paths(hip_device):
         codon.set_category_model(0, q[:4, :4], np.full(4, 0.25))
     lib = _lib.lib()
     assert lib.b3h_set_classes(None, None) != 0
-    assert lib.b3h_propose_lengths(None, 0, np.zeros(1, np.uint32), np.zeros(2, np.uint32),
-                                   np.zeros(2)) != 0
+    z1, z2, zd = np.zeros(1, np.uint32), np.zeros(2, np.uint32), np.zeros(2)
+    assert lib.b3h_propose_lengths(None, 0, _lib.addr(z1), _lib.addr(z2), _lib.addr(zd)) != 0
     assert C.c_int(lib.b3h_set_category_model(None, 0, np.zeros(16), np.ones(4), 0)).value != 0
