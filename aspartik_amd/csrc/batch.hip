@@ -45,9 +45,14 @@ size_t up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 struct BInst {
   uint32_t L = 0, I = 0, P = 0, cstride = 0;
   std::vector<uint8_t> editbuf, editbuf_next;
+  // per partial slot: 1 while the slot holds a cherry (both children
+  // leaves), whose partial is not stored but recomputed by its readers from
+  // the two leaves' codes and the matrices recorded for the slot
+  std::vector<uint8_t> slot_cherry, cherry_next;
   Scheduler sched;
   // offsets into the device arenas (element units of each arena)
   size_t part_off = 0, code_off = 0, site_off = 0, pat_off = 0, bsum_off = 0, pm_off = 0;
+  size_t cm_off = 0, cl_off = 0;  // cherry matrices (floats) and leaves (uint2) of its slots
   // offsets into the staging buffer (bytes)
   size_t st_entries = 0, st_times = 0, st_rates = 0;
   bool model_set = false;
@@ -75,6 +80,8 @@ struct b3h_batch {
   uint32_t *counters = nullptr;  // one per instance, 16 words apart
   double *totals = nullptr;
   float *pmats = nullptr;
+  float *cmats = nullptr;    // [slots][2][K][5][4] per instance
+  uint2 *cleaves = nullptr;  // [slots] per instance
   DevModel *models = nullptr;
   uint32_t *queue = nullptr;
 
@@ -117,7 +124,7 @@ void batch_release(b3h_batch *b) {
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   void *ptrs[] = {b->partials, b->exps,  b->codes,    b->site,   b->mixed,
                   b->weights,  b->blocksum, b->counters, b->totals, b->pmats,
-                  b->models,   b->queue, b->d_stage};
+                  b->cmats,    b->cleaves, b->models,  b->queue,  b->d_stage};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->h_stage) (void)hipHostFree(b->h_stage);
@@ -186,7 +193,7 @@ b3h_status b3h_batch_create(const b3h_config *cfg, uint32_t count, const b3h_ins
 
   // layout
   b->inst.resize(count);
-  size_t part = 0, code = 0, site = 0, pat = 0, bsum = 0, pm = 0, st = 0;
+  size_t part = 0, code = 0, site = 0, pat = 0, bsum = 0, pm = 0, st = 0, cm = 0, cl = 0;
   uint32_t max_I = 0;
   const uint32_t tp_max = 64u * b->T * 2u;
   for (uint32_t i = 0; i < count; i++) {
@@ -208,6 +215,12 @@ b3h_status b3h_batch_create(const b3h_config *cfg, uint32_t count, const b3h_ins
     bsum += (in.P + 64u * b->T - 1) / (64u * b->T);
     in.pm_off = pm;
     pm += (size_t)in.I * 2 * K * 20;  // [entry][side][K][5][4]: columns, the fifth all ones
+    const size_t slots = (size_t)(b->single ? 1 : 2) * in.I;
+    in.slot_cherry.assign(slots, 0);
+    in.cm_off = cm;
+    cm += slots * 2 * K * 20;
+    in.cl_off = cl;
+    cl += slots;
     max_I = std::max(max_I, in.I);
   }
   b->max_items = (uint32_t)bsum;
@@ -246,6 +259,8 @@ b3h_status b3h_batch_create(const b3h_config *cfg, uint32_t count, const b3h_ins
   if ((s = balloc(b, (void **)&b->counters, (size_t)count * 64))) return bail(s);
   if ((s = balloc(b, (void **)&b->totals, (size_t)count * 8))) return bail(s);
   if ((s = balloc(b, (void **)&b->pmats, pm * 4))) return bail(s);
+  if ((s = balloc(b, (void **)&b->cmats, cm * 4))) return bail(s);
+  if ((s = balloc(b, (void **)&b->cleaves, cl * 8))) return bail(s);
   if ((s = balloc(b, (void **)&b->models, (size_t)count * sizeof(DevModel)))) return bail(s);
   if ((s = balloc(b, (void **)&b->queue, 64))) return bail(s);
   if ((s = balloc(b, (void **)&b->d_stage, b->st_bytes))) return bail(s);
@@ -277,6 +292,8 @@ b3h_status b3h_batch_create(const b3h_config *cfg, uint32_t count, const b3h_ins
        hipMemset(b->queue, 0, 64) == hipSuccess &&
        hipMemset(b->exps, 0, part * 2) == hipSuccess &&
        hipMemset(b->partials, 0, part * 16) == hipSuccess &&
+       hipMemset(b->cmats, 0, cm * 4) == hipSuccess &&
+       hipMemset(b->cleaves, 0, cl * 8) == hipSuccess &&
        hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
     host_fail(B3H_ERR_HIP, "initial device upload failed");
@@ -363,12 +380,21 @@ b3h_status b3h_batch_propose_times(b3h_batch *b, const b3h_update *ups, uint32_t
       }
       eb = in.editbuf_next.data();
     }
+    // the new slot of every listed node holds a cherry iff both its children
+    // are leaves (the schedule build clears the flag of a cherry it has to
+    // store); the flags are committed with the EditBuf below
+    in.cherry_next = in.slot_cherry;
+    for (uint32_t j = 0; j < u.n; j++)
+      in.cherry_next[Scheduler::slot_of(u.internals[j], in.L, eb)] =
+          (u.children[2 * j] < in.L && u.children[2 * j + 1] < in.L) ? 1 : 0;
     std::string err;
-    if (!in.sched.build(u.n, u.internals, u.children, in.L, eb, true, b->cap, &err))
+    if (!in.sched.build(u.n, u.internals, u.children, in.L, eb, true, b->cap, &err,
+                        in.cherry_next.data()))
       return host_fail(B3H_ERR_ARG, "instance " + std::to_string(i) + ": " + err);
     max_n = std::max(max_n, u.n);
     max_slots = std::max(max_slots, in.sched.slots_used);
-    hbm_lists += in.sched.hbm_sources * 8 > u.n;
+    // stored cherries are recomputed only by the HBM-prefetch variant
+    hbm_lists += in.sched.hbm_sources * 8 > u.n || in.sched.cherry_sources > 0;
     active++;
   }
   if (active == 0) return host_fail(B3H_ERR_ARG, "no instance has an update");
@@ -384,6 +410,7 @@ b3h_status b3h_batch_propose_times(b3h_batch *b, const b3h_update *ups, uint32_t
     if (u.n == 0) continue;
     order.push_back(i);
     if (!b->single) in.editbuf.swap(in.editbuf_next);
+    in.slot_cherry.swap(in.cherry_next);
     std::memcpy(b->h_stage + in.st_entries, in.sched.entries.data(), (size_t)u.n * sizeof(Entry));
     std::memcpy(b->h_stage + in.st_times, u.edge_times, (size_t)2 * u.n * 8);
     std::memcpy(b->h_stage + in.st_rates, u.category_rates, (size_t)K * 8);
@@ -393,6 +420,8 @@ b3h_status b3h_batch_propose_times(b3h_batch *b, const b3h_update *ups, uint32_t
     pd[i].rates = reinterpret_cast<const double *>(b->d_stage + in.st_rates);
     pd[i].model = b->models + i;
     pd[i].out = b->pmats + in.pm_off;
+    pd[i].cmats = b->cmats + in.cm_off;
+    pd[i].cleaves = b->cleaves + in.cl_off;
     F32Args &a = args[i];
     a.entries = d_entries;
     a.pmats = b->pmats + in.pm_off;
@@ -410,8 +439,8 @@ b3h_status b3h_batch_propose_times(b3h_batch *b, const b3h_update *ups, uint32_t
     a.seq = 0;
     a.pik = nullptr;
     a.classes = nullptr;
-    a.cmats = nullptr;  // batched instances store every partial, cherries included
-    a.cleaves = nullptr;
+    a.cmats = b->cmats + in.cm_off;
+    a.cleaves = b->cleaves + in.cl_off;
     a.ln_inv_k = std::log(1.0 / (double)K);
     a.n = u.n;
     a.P = in.P;

@@ -34,6 +34,11 @@ struct PbuildDesc {
   const double *rates;
   const DevModel *model;
   float *out;
+  // or null: an entry whose children are both leaves also writes its
+  // matrices to cmats[out slot] ([2][K][5][4]) and its leaves to
+  // cleaves[out slot], as launch_pbuild does
+  float *cmats;
+  uint2 *cleaves;
   uint32_t n;
   uint32_t pad;
 };

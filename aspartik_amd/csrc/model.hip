@@ -235,13 +235,22 @@ __global__ void pbuild_batch_kernel(const PbuildDesc *desc, uint32_t K, uint32_t
     const uint64_t es = mat / K;
     const uint32_t side = (uint32_t)(es & 1);
     const uint32_t e = (uint32_t)(es >> 1);
-    const uint32_t orig = entry_index(d.entries[e].meta);
+    const Entry en = d.entries[e];
+    const uint32_t orig = entry_index(en.meta);
     const double t = d.times[2 * orig + side] * d.rates[c];
     // columns, the fifth all ones: [entry][side][K][5][4] (S = 4)
     const uint32_t r = el / 4, col = el % 4;
     float *o = d.out + mat * 20;
-    o[col * 4 + r] = (float)p_element(d.model, (int)S, t, (int)r, (int)col);
+    const float v = (float)p_element(d.model, (int)S, t, (int)r, (int)col);
+    o[col * 4 + r] = v;
     if (col == 0) o[16 + r] = 1.0f;
+    if (d.cmats && (en.left & SRC_TAG) == SRC_TIP && (en.right & SRC_TAG) == SRC_TIP) {
+      float *cm = d.cmats + (((uint64_t)en.out * 2 + side) * K + c) * 20;
+      cm[col * 4 + r] = v;
+      if (col == 0) cm[16 + r] = 1.0f;
+      if (side == 0 && c == 0 && el == 0)
+        d.cleaves[en.out] = make_uint2(en.left & SRC_IDX, en.right & SRC_IDX);
+    }
   }
 }
 

@@ -179,7 +179,7 @@ def bench_batch(args):
     every instance fully evaluated in one persistent launch per step,
     replayed from a captured HIP graph."""
     from aspartik_amd import workloads
-    from aspartik_amd.b3 import Batch, DatedTree, autotune_batch
+    from aspartik_amd.b3 import Batch, Calculator, DatedTree, autotune_batch
 
     log("[bench] building batched workload")
     wl = workloads.hky_gamma4_batch(count=args.instances)
@@ -190,11 +190,15 @@ def bench_batch(args):
         t = DatedTree(x["L"], x["parents"], x["children"], x["heights"])
         internals, children = workloads.full_lists(t)
         ups.append((internals, children, workloads.edge_times(t, children), rates))
-        B += workloads.store_bytes(internals, children, x["L"], K, x["P"])
+        B += workloads.store_bytes(internals, children, x["L"], K, x["P"], tree_children=t.children)
     data = [(x["codes"], None) for x in insts]
     models = [(x["model"].q_matrix(), x["model"].frequencies(), True) for x in insts]
-    log("[bench] batched: tuning tiles")
-    (tile, lanes), trials = autotune_batch(data, models, ups, K, evals=6)
+    if args.batch_tile:
+        tile, lanes = (int(v) for v in args.batch_tile.split("x"))
+        trials = []
+    else:
+        log("[bench] batched: tuning tiles")
+        (tile, lanes), trials = autotune_batch(data, models, ups, K, evals=6)
     for tr in trials:
         log(f"[bench]   tile {tr.tile_patterns} x {tr.lane_patterns}/lane: {tr.ms:.2f} ms per "
             f"batch, kernel {tr.kernel_ms:.2f} ms")
@@ -219,8 +223,36 @@ def bench_batch(args):
         b.accept()
     kms, kn = b.profile_read()
     b.profile_enable(False)
+    device_bytes = b.device_bytes()
     kernel_ms = kms / max(1, kn)
     log(f"[bench] batched: {args.steps / dt:.2f} batch evals/s, kernel {kernel_ms:.2f} ms")
+    b.close()
+    # the plain FP64 GPU path over the same instances, one at a time (both
+    # of its slots for every instance at once would exceed the HBM): each
+    # instance is created, evaluated once untimed and then timed over
+    # `f64_evals` full evaluations; the batch's FP64 time is the sum
+    f64_evals, f64_s, f64_err = 2, 0.0, 0.0
+    if not args.only_f32:
+        log("[bench] batched: plain FP64 GPU path, instance by instance")
+    for i, x in enumerate([] if args.only_f32 else insts):
+        c64 = Calculator.HIPPlainF64().make(x["codes"], x["P"], K)
+        c64.set_weights(np.ones(x["P"], np.uint32))
+        c64.set_model(*models[i])
+        c64.propose_times(*ups[i])
+        ref64 = c64.total()
+        c64.accept()
+        f64_err = max(f64_err, abs(totals[i] - ref64) / abs(ref64))
+        t1 = time.perf_counter()
+        for _ in range(f64_evals):
+            c64.propose_times(*ups[i])
+            c64.total()
+            c64.accept()
+        f64_s += (time.perf_counter() - t1) / f64_evals
+        c64.close()
+    if f64_s > 0:
+        log(f"[bench] batched: FP64 plain {f64_s * 1e3:.1f} ms per batch, FP32 batched "
+            f"speed-up {f64_s / (dt / args.steps):.2f}x, max total rel err vs FP64 "
+            f"{f64_err:.2e}")
     cpu, err = None, None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -272,10 +304,18 @@ def bench_batch(args):
                    "tile_patterns": tile, "lane_patterns": lanes},
         "roofline": {"bound": "hbm", "achieved": B / (kernel_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": B / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+                     "frac": B / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": read_traffic(f"hky_gamma4_batch{args.instances}",
+                                             "prune_f32_batch")},
         "cpu_baseline": cpu, "kernel_ms": kernel_ms, "algorithmic_bytes_per_eval": B,
         "batch_evals_per_s": args.steps / dt, "error_vs_fp64_oracle": err,
-        "device_bytes": b.device_bytes(),
+        "fp64_plain_gpu": None if f64_s == 0 else {"ms_per_batch": f64_s * 1e3,
+                           "instance_evals_per_s": len(insts) / f64_s,
+                           "speedup_of_batched_fp32": f64_s / (dt / args.steps),
+                           "max_total_rel_err_fp32_vs_fp64": f64_err,
+                           "method": f"each instance alone on the plain FP64 kernel, "
+                                     f"{f64_evals} timed full evaluations after one untimed"},
+        "device_bytes": device_bytes,
         "tile_trials": [{"tile_patterns": tr.tile_patterns, "lane_patterns": tr.lane_patterns,
                          "ms_per_batch": tr.ms, "kernel_ms": tr.kernel_ms} for tr in trials],
     }), flush=True)
@@ -851,6 +891,9 @@ def main():
     ap.add_argument("--codon-taxa", type=int, default=64)
     ap.add_argument("--codon-patterns", type=int, default=20_000)
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--batch-tile", default="",
+                    help="batched workload: tile shape PxL (patterns per workgroup x per lane) "
+                         "instead of the autotuner's choice")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
