@@ -101,29 +101,63 @@ __device__ __forceinline__ double pair_sum(double v) {
 
 }  // namespace
 
-// All LDS is static (77 KiB: matrices of both sides, per-wavefront child
-// tiles, codes and exponents, the one-hot table, frequencies); distinct
-// arrays let the compiler see that the LDS-direct prefetches touch none of
-// the others.  Two workgroups fit a CU.
+// All LDS is static (32.5 KiB: the matrices of both sides and the root
+// frequencies); with at most 168 VGPRs three 4-wavefront workgroups fit a
+// CU, so 20,000 patterns x 4 categories (628 workgroups) run in one round on
+// the 256 CUs.
 size_t prune_codon_lds_bytes(uint32_t) { return 0; }
+
+namespace {
+
+// Buffer resource over `bytes` bytes at a wave-uniform base: a resource of
+// 0 bytes makes every lane's access out of range, which returns zeros and
+// moves no data, so a prefetch of an absent source costs an instruction but
+// no memory traffic, and every node issues the same loads.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, bytes, 0x00020000);
+}
+
+// B operand of a tip: the lane's 32 states (register order) one-hot at
+// `code`.  State row(h, 8 s + m) = 16 s + 8 (m / 4) + 4 h + m % 4, so code k
+// sits in the half of lane-half (k >> 2) & 1, at register position
+// 8 (k >> 4) + 4 ((k >> 3) & 1) + (k & 3); codes >= 64 give zeros.
+__device__ __forceinline__ Tile one_hot(uint32_t code, uint32_t h) {
+  const uint32_t hit = (code < 64u && ((code >> 2) & 1u) == h)
+                           ? 8u * (code >> 4) + 4u * ((code >> 3) & 1u) + (code & 3u)
+                           : 64u;
+  const uint32_t word = hit >> 1, val = 0x3C00u << (16u * (hit & 1u));  // FP16 1.0
+  Tile b;
+#pragma unroll
+  for (uint32_t s = 0; s < 4; s++) {
+    u4 w;
+#pragma unroll
+    for (uint32_t d = 0; d < 4; d++) w[d] = (4u * s + d == word) ? val : 0u;
+    b.v[s] = __builtin_bit_cast(h8, w);
+  }
+  return b;
+}
+
+}  // namespace
 
 // grid (Ppad / 128, K): blockIdx.y is the rate category; 4 wavefronts per
 // workgroup, each owning one 32-pattern tile (the pattern count is padded to
 // a multiple of 128, so every wavefront's tile exists and every access of
 // the padded codes, partials and exponents is in bounds).
 //
-// Per node i the global memory operations are issued in a fixed order on
-// every path, so that each wait covers only what it needs:
-//   node start   12 LDS-direct loads: entry i+1's tip codes (2) and the tiles
-//                (4 + 4) and exponents (1 + 1) of its HBM children; absent
-//                sources read a shared dummy tile so that the count never
-//                varies
-//   barrier      (after the products) store entry i+1's matrices (loaded
-//                during node i-1), copy entry i+1's prefetched sources from
-//                LDS to registers, load entry i+2's matrices
-//   node end     the node's partial (4 stores) and exponent (1 store), then
-//                a barrier that publishes entry i+1's matrices
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+// Per node i the global memory operations are issued in a fixed order:
+//   node start   7 buffer loads, the sources of entry i+1 into registers:
+//                one stored child's tile (4) and exponent (1), and the tip
+//                codes of both sides (2); a load whose source is absent has
+//                an empty range
+//   node end     (after the products and the normalisation) 8 loads of
+//                entry i+1's matrices, the node's partial (4 stores) and
+//                exponent (1 store), barrier 1, the matrices written to LDS,
+//                barrier 2
+// Entry i+1's sources thus load during node i's products, its matrices
+// during node i's stores and first barrier.  A second stored child of one
+// entry (two children read from HBM: the first entry of some partial
+// updates) is loaded when the entry starts.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void
 prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u, h = lane / 32u, col = lane % 32u;
@@ -133,36 +167,20 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
   const uint32_t tile = blockIdx.x * kWaves + t;
   const uint32_t p = tile * 32u + col;
   const size_t tiles_all = a.Ppad / 32u;
-  // [s][lane][8] tile of (slot, category), and the lane's exponent
-  _Float16 *const tile_base = a.partials + ((size_t)c * tiles_all + tile) * 2048u + lane * 8u;
-  int16_t *const exp_base = a.exps + (size_t)c * a.Ppad + p;
+  // [s][lane][8] tile of (slot, category) and the tile's 32 exponents, at
+  // wave-uniform bases
   const size_t tile_stride = (size_t)K * tiles_all * 2048u, exp_stride = (size_t)K * a.Ppad;
-  auto tile_ptr = [&](uint32_t slot) -> _Float16 * { return tile_base + slot * tile_stride; };
-  auto exp_ptr = [&](uint32_t slot) -> int16_t * { return exp_base + slot * exp_stride; };
+  _Float16 *const tile0 = a.partials + ((size_t)c * tiles_all + tile) * 2048u;
+  int16_t *const exp0 = a.exps + (size_t)c * a.Ppad + tile * 32u;
+  const uint8_t *const code0 = a.codes + (size_t)tile * 32u;
 
   __shared__ double pi_s[64];
   __shared__ __align__(16) _Float16 amat0[2 * kMatH];  // side 0: hi, lo
   __shared__ __align__(16) _Float16 amat1[2 * kMatH];  // side 1
-  // LDS-direct prefetch targets, per wavefront and side: child tile
-  // [s][lane][8]; codes and exponents land one dword per lane (only the first
-  // 32 bytes, resp. 64 bytes, are the wavefront's patterns)
-  __shared__ __align__(16) _Float16 tbuf[kWaves][2][2048];
-  __shared__ __align__(16) uint32_t cbuf[kWaves][2][64];
-  __shared__ __align__(16) uint32_t ebuf[kWaves][2][64];
-  // B operands of tips: hot[code][h] is the lane's 32 states (register
-  // order) one-hot at `code`; code 64 (any code >= 64) is all zeros.  An
-  // unknown code's product is replaced by the all-ones column anyway.
-  __shared__ __align__(16) _Float16 hot[65][2][32];
   if (tid < 64) pi_s[tid] = a.pi[tid];
-  for (uint32_t e = tid; e < 65u * 64u; e += 256u) {
-    const uint32_t code = e / 64u, hh = (e / 32u) & 1u, r = e % 32u;
-    hot[code][hh][r] = row_of(hh, r) == code ? (_Float16)1.0f : (_Float16)0.0f;
-  }
 
   // matrices of entry e, category c: 2 sides x 2 parts x 64 x 64 halves =
-  // 2048 units of 16 B, eight per thread of the 256-thread workgroup, loaded
-  // into registers one node ahead and stored once every wavefront is done
-  // with the current node's.
+  // 2048 units of 16 B, eight per thread of the 256-thread workgroup
   u4 m[8];
   auto fetch_mats = [&](uint32_t e) {
 #pragma unroll
@@ -188,78 +206,79 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
     }
   };
 
-  // Sources of entry i into LDS.  Tip rows are read 4 bytes per lane at
-  // pattern offset 4 (lane % 8); exponents 2 patterns per lane at offset
-  // 2 (lane % 16).  A side that is not a tip reads tip row 0, and a side that
-  // is not an HBM child reads the tile and exponents of (slot 0, category 0,
-  // tile 0), the same few lines for every wavefront.
-  const _Float16 *const dummy_tile = a.partials + lane * 8u;
-  const int16_t *const dummy_exp = a.exps + 2u * (lane % 16u);
-  auto prefetch = [&](uint32_t i) {
-    const uint4 e4 = ents[i];
-    const uint32_t src[2] = {e4.y, e4.z};
-    const uint8_t *crow = a.codes + (size_t)tile * 32u + 4u * (lane % 8u);
-#pragma unroll
-    for (uint32_t side = 0; side < 2; side++) {
-      const uint32_t tag = src[side] & SRC_TAG, idx = src[side] & SRC_IDX;
-      __builtin_amdgcn_global_load_lds(crow + (size_t)(tag == SRC_TIP ? idx : 0u) * a.cstride,
-                                       &cbuf[t][side][0], 4, 0, 0);
-      const _Float16 *g = tag == SRC_HBM ? tile_ptr(idx) : dummy_tile;
-      const int16_t *ge = tag == SRC_HBM ? exp_ptr(idx) - col + 2u * (lane % 16u) : dummy_exp;
-#pragma unroll
-      for (uint32_t s = 0; s < 4; s++)
-        __builtin_amdgcn_global_load_lds(g + s * 512u, &tbuf[t][side][s * 512u], 16, 0, 0);
-      __builtin_amdgcn_global_load_lds(ge, &ebuf[t][side][0], 4, 0, 0);
-    }
-  };
-  // the prefetched sources of the next entry, in registers
-  Tile y[2];
+  // Sources of entry i in registers: the first stored child's tile and
+  // exponent (yside: its side, 2 when there is none) and both sides' codes.
+  Tile y;
+  int ey = 0;
+  uint32_t yside = 2u;
   uint32_t code[2] = {0u, 0u};
-  int ey[2] = {0, 0};
-  auto take = [&]() {
+  // (issued for every entry, the last one included, with empty ranges when
+  // there is no next entry: a conditional load would make the compiler's
+  // waits for the other loads conservative)
+  auto prefetch = [&](uint32_t i, bool live) {
+    const uint4 e4 = live ? ents[i] : make_uint4(0u, SRC_PREV, SRC_PREV, 0u);
+    const uint32_t src[2] = {e4.y, e4.z};
+    const bool hbm0 = (src[0] & SRC_TAG) == SRC_HBM, hbm1 = (src[1] & SRC_TAG) == SRC_HBM;
+    yside = hbm0 ? 0u : hbm1 ? 1u : 2u;
+    const uint32_t slot = (hbm0 ? src[0] : src[1]) & SRC_IDX;
+    const bool any = yside < 2u;
+    const auto rt = rsrc(tile0 + (any ? slot : 0u) * tile_stride, any ? 4096u : 0u);
+    const auto re = rsrc(exp0 + (any ? slot : 0u) * exp_stride, any ? 64u : 0u);
+#pragma unroll
+    for (uint32_t s = 0; s < 4; s++)
+      y.v[s] = __builtin_bit_cast(
+          h8, __builtin_amdgcn_raw_buffer_load_b128(rt, (int)((s * 64u + lane) * 16u), 0, 0));
+    ey = exp_unpack((int16_t)__builtin_amdgcn_raw_buffer_load_b16(re, (int)(col * 2u), 0, 0));
 #pragma unroll
     for (uint32_t side = 0; side < 2; side++) {
-#pragma unroll
-      for (uint32_t s = 0; s < 4; s++)
-        y[side].v[s] = *reinterpret_cast<const h8 *>(&tbuf[t][side][s * 512u + lane * 8u]);
-      code[side] = reinterpret_cast<const uint8_t *>(&cbuf[t][side][0])[col];
-      ey[side] = exp_unpack(reinterpret_cast<const int16_t *>(&ebuf[t][side][0])[col]);
+      const bool tip = (src[side] & SRC_TAG) == SRC_TIP;
+      const auto rc = rsrc(code0 + (size_t)(tip ? (src[side] & SRC_IDX) : 0u) * a.cstride,
+                           tip ? 32u : 0u);
+      code[side] = __builtin_amdgcn_raw_buffer_load_b8(rc, (int)col, 0, 0);
     }
   };
 
   fetch_mats(0);
-  __syncthreads();  // frequencies, one-hot table
+  prefetch(0, true);
   store_mats();
-  prefetch(0);
-  if (n > 1) fetch_mats(1);
-  __syncthreads();  // matrices of entry 0, prefetched sources of entry 0
-  take();
+  __syncthreads();  // frequencies, matrices of entry 0
 
-  int E = 0;
+  Tile x;  // the previous entry's result (FP16, stored layout)
+  int ex_prev = 0;
   for (uint32_t i = 0; i < n; i++) {
     const uint4 e4 = ents[i];
     const uint32_t out = e4.x, srcs[2] = {e4.y, e4.z};
     const bool more = i + 1 < n;
-    if (more) prefetch(i + 1);
 
-    // B operands and exponents of both children: a tip's one-hot column, or
-    // y[side], which holds the HBM child or (put there at the end of the
-    // previous node) the previous result
+    // B operands and exponents of both children: a tip's one-hot column,
+    // the previous result, or a stored child (prefetched, or loaded here
+    // when it is the entry's second stored child)
     Tile b[2];
     int ce[2];
+    // this entry's codes: the prefetch below overwrites code[]
+    const uint32_t cd[2] = {code[0], code[1]};
 #pragma unroll
     for (uint32_t side = 0; side < 2; side++) {
-      if ((srcs[side] & SRC_TAG) == SRC_TIP) {
-        const uint32_t cd = code[side];
-        const _Float16 *hb = &hot[cd < 64u ? cd : 64u][h][0];
-#pragma unroll
-        for (uint32_t s = 0; s < 4; s++) b[side].v[s] = *reinterpret_cast<const h8 *>(hb + 8 * s);
+      const uint32_t tag = srcs[side] & SRC_TAG;
+      if (tag == SRC_TIP) {
+        b[side] = one_hot(cd[side], h);
         ce[side] = 0;
+      } else if (tag == SRC_PREV) {
+        b[side] = x;
+        ce[side] = ex_prev;
+      } else if (side == yside) {
+        b[side] = y;
+        ce[side] = ey;
       } else {
-        b[side] = y[side];
-        ce[side] = ey[side];
+        const uint32_t slot = srcs[side] & SRC_IDX;
+        const h8 *g = reinterpret_cast<const h8 *>(tile0 + slot * tile_stride) + lane;
+#pragma unroll
+        for (uint32_t s = 0; s < 4; s++) b[side].v[s] = g[64u * s];
+        ce[side] = exp_unpack(exp0[slot * exp_stride + col]);
       }
     }
+    prefetch(i + 1, more);
+
     // per row block, the two children's chains interleaved step by step:
     // lo parts, the 2^-20 scale, then hi parts accumulated on top; unknown
     // tips take the all-ones column (row_of(h, i) = row_of(0, i) + 4 h)
@@ -286,7 +305,7 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
           r[1] *= kLoScale;
         }
       }
-      // issue order of the row block: two A-fragment reads ahead, then one
+      // issue order of the row block: four A-fragment reads ahead, then one
       // read per MFMA, so that no MFMA waits out a whole LDS round trip
       __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
 #pragma unroll
@@ -297,8 +316,8 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
       __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
 #pragma unroll
       for (uint32_t side = 0; side < 2; side++) {
-        if ((srcs[side] & SRC_TAG) == SRC_TIP && __builtin_amdgcn_ballot_w64(code[side] >= S)) {
-          const bool unk = code[side] >= S;
+        if ((srcs[side] & SRC_TAG) == SRC_TIP && __builtin_amdgcn_ballot_w64(cd[side] >= S)) {
+          const bool unk = cd[side] >= S;
           // opaque here, so that the 16 selects are made in this (rare)
           // branch instead of being hoisted out of the node loop as 32
           // registers of constants held for the whole kernel
@@ -310,12 +329,9 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
         }
       }
       acc[rb] = r[0] * r[1];
-    }
-    if (more) {
-      lds_barrier();  // every wavefront is done with this node's matrices
-      store_mats();   // the next entry's
-      take();
-      if (i + 2 < n) fetch_mats(i + 2);
+      // the product now, so that the first row block's two accumulators are
+      // not held through the second's
+      __builtin_amdgcn_sched_barrier(0);
     }
     // product and per-pattern normalisation across the lane pair
     float v[32];
@@ -330,29 +346,28 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
     const int ex = mx > 0.0f ? __builtin_amdgcn_frexp_expf(mx) : 0;
 #pragma unroll
     for (uint32_t r = 0; r < 32; r++) v[r] = __builtin_amdgcn_ldexpf(v[r], kXTop - ex);
-    E = exp_clamp(ce[0] + ce[1] + ex - kXTop);
+    const int E = exp_clamp(ce[0] + ce[1] + ex - kXTop);
 
     if (more) {
-      Tile x;
 #pragma unroll
       for (uint32_t s = 0; s < 4; s++)
 #pragma unroll
         for (uint32_t q = 0; q < 8; q++) x.v[s][q] = (_Float16)v[8 * s + q];
-      const uint4 en = ents[i + 1];
-#pragma unroll
-      for (uint32_t side = 0; side < 2; side++)
-        if (((side == 0 ? en.y : en.z) & SRC_TAG) == SRC_PREV) {
-          y[side] = x;
-          ey[side] = E;
-        }
-      // streaming stores: keep the L2 for the matrices every workgroup reads
-      // (both halves of a pattern store the same exponent)
-      u4 *g = reinterpret_cast<u4 *>(tile_ptr(out));
+      ex_prev = E;
+      // the next entry's matrices (loaded only now: held through the
+      // products they would need more registers than three waves per SIMD
+      // leave), then streaming stores, which keep the L2 for the matrices
+      // every workgroup reads (both halves of a pattern store the same
+      // exponent)
+      fetch_mats(i + 1);
+      u4 *g = reinterpret_cast<u4 *>(tile0 + out * tile_stride) + lane;
 #pragma unroll
       for (uint32_t s = 0; s < 4; s++)
         __builtin_nontemporal_store(__builtin_bit_cast(u4, x.v[s]), g + s * 64u);
-      __builtin_nontemporal_store(exp_pack(E), exp_ptr(out));
-      lds_barrier();  // the next entry's matrices are in LDS
+      __builtin_nontemporal_store(exp_pack(E), exp0 + out * exp_stride + col);
+      lds_barrier();  // every wavefront is done with this node's matrices
+      store_mats();   // the next entry's
+      lds_barrier();  // ... visible to every wavefront
     } else {
       // root of the update: ln(sum_j pi_j x_j) + E ln 2 in FP64 from the
       // unrounded FP32 values (the root partial is never read as a child)
