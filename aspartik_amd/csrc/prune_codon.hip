@@ -179,30 +179,29 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
   __shared__ __align__(16) _Float16 amat1[2 * kMatH];  // side 1
   if (tid < 64) pi_s[tid] = a.pi[tid];
 
-  // matrices of entry e, category c: 2 sides x 2 parts x 64 x 64 halves =
-  // 2048 units of 16 B, eight per thread of the 256-thread workgroup
-  u4 m[8];
-  auto fetch_mats = [&](uint32_t e) {
+  // Matrices of entry e, category c: 2 sides x 2 parts x 64 x 64 halves,
+  // loaded straight into LDS (global_load_lds, no registers): load q of
+  // wavefront t fills the 1 KiB of rows 8 t .. 8 t + 7 of part q / 2, row
+  // block q % 2, one 16-byte unit per lane, so lane l writes row
+  // r = 32 (q % 2) + 8 t + l / 8, chunk slot l % 8, which holds logical chunk
+  // (l % 8) ^ ((r >> 1) & 7); the lane's source address is chosen to match.
+  // (the lane's part of the source address is one 32-bit byte offset from a
+  // wave-uniform base)
+  const uint32_t mrow = 8u * t + (lane >> 3);
+  const uint32_t moff = mrow * 128u + 16u * ((lane & 7u) ^ ((mrow >> 1) & 7u));
+  auto dma_mats = [&](uint32_t e) {
 #pragma unroll
     for (uint32_t side = 0; side < 2; side++) {
-      const u4 *src =
-          reinterpret_cast<const u4 *>(a.amats + ((size_t)(2 * e + side) * K + c) * 8192u) + tid;
+      const char *src = reinterpret_cast<const char *>(
+          a.amats + ((size_t)(2 * e + side) * K + c) * 8192u);
+      _Float16 *dst = side == 0 ? amat0 : amat1;
 #pragma unroll
-      for (uint32_t q = 0; q < 4; q++) m[4 * side + q] = src[256u * q];
-    }
-  };
-  // unit w = tid + 256 q: part w / 512, row (w / 8) % 64, 16-byte chunk w % 8
-  // (rows srow and srow + 32 share the swizzle)
-  const uint32_t srow = (tid >> 3) & 31u;
-  const uint32_t soff = srow * 64u + 8u * ((tid & 7u) ^ ((srow >> 1) & 7u));
-  auto store_mats = [&]() {
-#pragma unroll
-    for (uint32_t side = 0; side < 2; side++) {
-      _Float16 *d = (side == 0 ? amat0 : amat1) + soff;
-      *reinterpret_cast<u4 *>(d) = m[4 * side + 0];
-      *reinterpret_cast<u4 *>(d + 32u * 64u) = m[4 * side + 1];
-      *reinterpret_cast<u4 *>(d + kMatH) = m[4 * side + 2];
-      *reinterpret_cast<u4 *>(d + kMatH + 32u * 64u) = m[4 * side + 3];
+      for (uint32_t q = 0; q < 4; q++) {
+        const uint32_t part = (q >> 1) * (2u * kMatH) + (q & 1u) * 4096u;  // bytes
+        __builtin_amdgcn_global_load_lds(
+            reinterpret_cast<const void *>(src + part + moff),
+            dst + (q >> 1) * kMatH + (32u * (q & 1u) + 8u * t) * 64u, 16, 0, 0);
+      }
     }
   };
 
@@ -238,10 +237,9 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
     }
   };
 
-  fetch_mats(0);
+  dma_mats(0);
   prefetch(0, true);
-  store_mats();
-  __syncthreads();  // frequencies, matrices of entry 0
+  __syncthreads();  // frequencies, matrices of entry 0 (waits for every load)
 
   Tile x;  // the previous entry's result (FP16, stored layout)
   int ex_prev = 0;
@@ -275,6 +273,9 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
 #pragma unroll
         for (uint32_t s = 0; s < 4; s++) b[side].v[s] = g[64u * s];
         ce[side] = exp_unpack(exp0[slot * exp_stride + col]);
+        // complete here (vmcnt(0)), so that the common path after the join
+        // does not wait for the previous entry's stores
+        __builtin_amdgcn_s_waitcnt(0x0F70);
       }
     }
     prefetch(i + 1, more);
@@ -333,6 +334,11 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
       // not held through the second's
       __builtin_amdgcn_sched_barrier(0);
     }
+    if (more) {
+      lds_barrier();    // every wavefront is done with this node's matrices
+      dma_mats(i + 1);  // the next entry's, in flight during the normalisation
+      __builtin_amdgcn_sched_barrier(0);
+    }
     // product and per-pattern normalisation across the lane pair
     float v[32];
     float mx = 0.0f;
@@ -354,20 +360,22 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
 #pragma unroll
         for (uint32_t q = 0; q < 8; q++) x.v[s][q] = (_Float16)v[8 * s + q];
       ex_prev = E;
-      // the next entry's matrices (loaded only now: held through the
-      // products they would need more registers than three waves per SIMD
-      // leave), then streaming stores, which keep the L2 for the matrices
-      // every workgroup reads (both halves of a pattern store the same
-      // exponent)
-      fetch_mats(i + 1);
+      // streaming stores, which keep the L2 for the matrices every
+      // workgroup reads (both halves of a pattern store the same exponent)
       u4 *g = reinterpret_cast<u4 *>(tile0 + out * tile_stride) + lane;
 #pragma unroll
       for (uint32_t s = 0; s < 4; s++)
         __builtin_nontemporal_store(__builtin_bit_cast(u4, x.v[s]), g + s * 64u);
       __builtin_nontemporal_store(exp_pack(E), exp0 + out * exp_stride + col);
-      lds_barrier();  // every wavefront is done with this node's matrices
-      store_mats();   // the next entry's
-      lds_barrier();  // ... visible to every wavefront
+      // this wavefront's matrix loads are complete once at most the five
+      // stores after them are outstanding (vector memory operations retire
+      // in issue order); then the next entry's matrices are visible to
+      // every wavefront
+      // (s_waitcnt vmcnt(5) expcnt(7) lgkmcnt(0) as a builtin, which the
+      // compiler's own wait insertion takes into account)
+      __builtin_amdgcn_sched_barrier(0);
+      __builtin_amdgcn_s_waitcnt(0x0075);
+      __builtin_amdgcn_s_barrier();
     } else {
       // root of the update: ln(sum_j pi_j x_j) + E ln 2 in FP64 from the
       // unrounded FP32 values (the root partial is never read as a child)
