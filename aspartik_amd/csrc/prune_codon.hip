@@ -42,8 +42,17 @@
 // LDS; the next node's are loaded into registers a node ahead and written
 // between the two barriers of a node.  Each wavefront runs its two
 // children's product chains interleaved, one 32-row block at a time.
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
+
+// Buffer load of `size` bytes per lane straight into LDS (buffer_load ...
+// lds): lane l's data lands at lds + l * size.
+typedef int32_t i4 __attribute__((ext_vector_type(4)));
+__device__ void buffer_load_lds(i4 rsrc, __attribute__((address_space(3))) uint32_t *lds,
+                                int size, int voffset, int soffset, int offset,
+                                int aux) __asm("llvm.amdgcn.raw.buffer.load.lds");
 
 namespace b3h {
 
@@ -117,6 +126,22 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, uint32_t b
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, bytes, 0x00020000);
 }
 
+// The same resource as four dwords (base, stride 0 | base high, bytes,
+// flags), the form buffer_load_lds takes.
+__device__ __forceinline__ i4 wave_rsrc(const void *p, uint32_t bytes) {
+  const uint64_t b = reinterpret_cast<uint64_t>(p);
+  i4 r;
+  r[0] = __builtin_amdgcn_readfirstlane((int32_t)(uint32_t)b);
+  r[1] = __builtin_amdgcn_readfirstlane((int32_t)((uint32_t)(b >> 32) & 0xFFFFu));
+  r[2] = (int32_t)bytes;
+  r[3] = 0x00020000;
+  return r;
+}
+__device__ __forceinline__ __attribute__((address_space(3))) uint32_t *lds_ptr(void *p) {
+  return reinterpret_cast<__attribute__((address_space(3))) uint32_t *>(
+      reinterpret_cast<uintptr_t>(p));
+}
+
 // B operand of a tip: the lane's 32 states (register order) one-hot at
 // `code`.  State row(h, 8 s + m) = 16 s + 8 (m / 4) + 4 h + m % 4, so code k
 // sits in the half of lane-half (k >> 2) & 1, at register position
@@ -185,35 +210,29 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
   // block q % 2, one 16-byte unit per lane, so lane l writes row
   // r = 32 (q % 2) + 8 t + l / 8, chunk slot l % 8, which holds logical chunk
   // (l % 8) ^ ((r >> 1) & 7); the lane's source address is chosen to match.
-  // (the lane's part of the source address is one 32-bit byte offset from a
-  // wave-uniform base)
+  // (the lane's part of the source address is one 32-bit byte offset; the
+  // rest, resource base and soffset, is wave-uniform)
   const uint32_t mrow = 8u * t + (lane >> 3);
   const uint32_t moff = mrow * 128u + 16u * ((lane & 7u) ^ ((mrow >> 1) & 7u));
   auto dma_mats = [&](uint32_t e) {
+    const i4 rs = wave_rsrc(a.amats + ((size_t)(2 * e) * K + c) * 8192u, (K + 1u) * 16384u);
 #pragma unroll
     for (uint32_t side = 0; side < 2; side++) {
-      const char *src = reinterpret_cast<const char *>(
-          a.amats + ((size_t)(2 * e + side) * K + c) * 8192u);
       _Float16 *dst = side == 0 ? amat0 : amat1;
 #pragma unroll
-      for (uint32_t q = 0; q < 4; q++) {
-        const uint32_t part = (q >> 1) * (2u * kMatH) + (q & 1u) * 4096u;  // bytes
-        __builtin_amdgcn_global_load_lds(
-            reinterpret_cast<const void *>(src + part + moff),
-            dst + (q >> 1) * kMatH + (32u * (q & 1u) + 8u * t) * 64u, 16, 0, 0);
-      }
+      for (uint32_t q = 0; q < 4; q++)
+        buffer_load_lds(rs, lds_ptr(dst + (q >> 1) * kMatH + (32u * (q & 1u) + 8u * t) * 64u), 16,
+                        (int)moff,
+                        (int)(side * K * 16384u + (q >> 1) * 8192u + (q & 1u) * 4096u), 0, 0);
     }
   };
 
   // Sources of entry i in registers: the first stored child's tile and
   // exponent (yside: its side, 2 when there is none) and both sides' codes.
   Tile y;
-  int ey = 0;
+  int16_t ey = 0;  // packed
   uint32_t yside = 2u;
   uint32_t code[2] = {0u, 0u};
-  // (issued for every entry, the last one included, with empty ranges when
-  // there is no next entry: a conditional load would make the compiler's
-  // waits for the other loads conservative)
   auto prefetch = [&](uint32_t i, bool live) {
     const uint4 e4 = live ? ents[i] : make_uint4(0u, SRC_PREV, SRC_PREV, 0u);
     const uint32_t src[2] = {e4.y, e4.z};
@@ -227,7 +246,9 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
     for (uint32_t s = 0; s < 4; s++)
       y.v[s] = __builtin_bit_cast(
           h8, __builtin_amdgcn_raw_buffer_load_b128(rt, (int)((s * 64u + lane) * 16u), 0, 0));
-    ey = exp_unpack((int16_t)__builtin_amdgcn_raw_buffer_load_b16(re, (int)(col * 2u), 0, 0));
+    // (unpacked where it is used: unpacking here would make the compiler
+    // wait for this load before the node's own work)
+    ey = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(re, (int)(col * 2u), 0, 0);
 #pragma unroll
     for (uint32_t side = 0; side < 2; side++) {
       const bool tip = (src[side] & SRC_TAG) == SRC_TIP;
@@ -243,10 +264,12 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
 
   Tile x;  // the previous entry's result (FP16, stored layout)
   int ex_prev = 0;
-  for (uint32_t i = 0; i < n; i++) {
+  // entry i; the root (the last entry) is a separate instance: it has no
+  // next entry, and its path does not join the others'
+  auto node = [&](uint32_t i, auto root_tag) {
+    constexpr bool more = !decltype(root_tag)::value;
     const uint4 e4 = ents[i];
     const uint32_t out = e4.x, srcs[2] = {e4.y, e4.z};
-    const bool more = i + 1 < n;
 
     // B operands and exponents of both children: a tip's one-hot column,
     // the previous result, or a stored child (prefetched, or loaded here
@@ -266,7 +289,7 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
         ce[side] = ex_prev;
       } else if (side == yside) {
         b[side] = y;
-        ce[side] = ey;
+        ce[side] = exp_unpack(ey);
       } else {
         const uint32_t slot = srcs[side] & SRC_IDX;
         const h8 *g = reinterpret_cast<const h8 *>(tile0 + slot * tile_stride) + lane;
@@ -278,7 +301,7 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
         __builtin_amdgcn_s_waitcnt(0x0F70);
       }
     }
-    prefetch(i + 1, more);
+    if constexpr (more) prefetch(i + 1, true);
 
     // per row block, the two children's chains interleaved step by step:
     // lo parts, the 2^-20 scale, then hi parts accumulated on top; unknown
@@ -334,7 +357,7 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
       // not held through the second's
       __builtin_amdgcn_sched_barrier(0);
     }
-    if (more) {
+    if constexpr (more) {
       lds_barrier();    // every wavefront is done with this node's matrices
       dma_mats(i + 1);  // the next entry's, in flight during the normalisation
       __builtin_amdgcn_sched_barrier(0);
@@ -354,7 +377,7 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
     for (uint32_t r = 0; r < 32; r++) v[r] = __builtin_amdgcn_ldexpf(v[r], kXTop - ex);
     const int E = exp_clamp(ce[0] + ce[1] + ex - kXTop);
 
-    if (more) {
+    if constexpr (more) {
 #pragma unroll
       for (uint32_t s = 0; s < 4; s++)
 #pragma unroll
@@ -362,11 +385,16 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
       ex_prev = E;
       // streaming stores, which keep the L2 for the matrices every
       // workgroup reads (both halves of a pattern store the same exponent)
-      u4 *g = reinterpret_cast<u4 *>(tile0 + out * tile_stride) + lane;
+      // (buffer stores: a wave-uniform base and a 32-bit lane offset; aux 2
+      // is the nontemporal hint)
+      const auto rt = rsrc(tile0 + out * tile_stride, 4096u);
 #pragma unroll
       for (uint32_t s = 0; s < 4; s++)
-        __builtin_nontemporal_store(__builtin_bit_cast(u4, x.v[s]), g + s * 64u);
-      __builtin_nontemporal_store(exp_pack(E), exp0 + out * exp_stride + col);
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, x.v[s]), rt,
+                                               (int)((s * 64u + lane) * 16u), 0, 2);
+      __builtin_amdgcn_raw_buffer_store_b16((uint16_t)exp_pack(E),
+                                            rsrc(exp0 + out * exp_stride, 64u), (int)(col * 2u),
+                                            0, 2);
       // this wavefront's matrix loads are complete once at most the five
       // stores after them are outstanding (vector memory operations retire
       // in issue order); then the next entry's matrices are visible to
@@ -386,7 +414,9 @@ prune_codon_kernel(CodonArgs a, const uint4 *__restrict__ ents) {
       sum = pair_sum(sum);
       if (p < a.P && h == 0) a.site[(size_t)c * a.P + p] = exp_site(sum, E);
     }
-  }
+  };
+  for (uint32_t i = 0; i + 1 < n; i++) node(i, std::false_type{});
+  node(n - 1, std::true_type{});
 }
 
 // the kernel uses static LDS only: nothing to raise
