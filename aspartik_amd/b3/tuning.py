@@ -65,11 +65,15 @@ def default_candidates(num_categories: int, device: int = 0):
 
 
 def autotune(codes, weights, tree: DatedTree, substitution, category_rates,
-             partial_fraction: float = 0.9, evals: int = 20, candidates=None, seed: int = 0):
+             partial_fraction: float = 0.9, evals: int = 20, candidates=None, seed: int = 0,
+             warm_s: float = 0.0):
     """Returns (best Calculator, [Trial, ...] sorted by score).
 
     partial_fraction: share of partial evaluations in the expected mix (an
-    MCMC chain over a fixed alignment is mostly partial updates)."""
+    MCMC chain over a fixed alignment is mostly partial updates).
+    warm_s: seconds of untimed full evaluations before each candidate's
+    timing (at least three are always run), so that every candidate is
+    timed at the device's steady clocks."""
     codes = np.ascontiguousarray(codes, np.uint8)
     P = codes.shape[1]
     rates = np.ascontiguousarray(category_rates, np.float64)
@@ -91,10 +95,13 @@ def autotune(codes, weights, tree: DatedTree, substitution, category_rates,
             t.mark_edge_updated(e)
             parts.append(_lists(t, cfg.propagations))
             t.accept()
-        for _ in range(3):
+        t_end = time.perf_counter() + warm_s
+        nwarm = 0
+        while nwarm < 3 or time.perf_counter() < t_end:
             _propose(calc, full, rates)
             calc.total()
             calc.accept()
+            nwarm += 1
         calc.profile_enable(True)
         t0 = time.perf_counter()
         for _ in range(evals):

@@ -651,6 +651,13 @@ def bench_hetero(args):
     for c, s in enumerate(subs):
         calc.set_category_model(c, s.q_matrix(), s.frequencies())
     calc.set_classes(classes)
+    # untimed evaluations for the pre-warm period, then the warm-up steps, so
+    # that the device runs at its steady clocks when timing starts
+    t_end = time.perf_counter() + args.prewarm
+    while time.perf_counter() < t_end:
+        calc.propose_lengths(*full, lengths)
+        calc.total()
+        calc.accept()
     for _ in range(args.warmup):
         calc.propose_lengths(*full, lengths)
         calc.total()
@@ -749,7 +756,7 @@ def bench_storage(args):
                                       Calculator.HIPPropagationsF64())]
     log(f"[bench] autotuning {len(cands)} candidates, {evals} evaluations each")
     best, trials = autotune(wl["codes"], wl["weights"], tree, wl["model"], wl["rates"],
-                            evals=evals, candidates=cands)
+                            evals=evals, candidates=cands, warm_s=0.5)
     table = [{"layout": t.name, "full_ms": t.full_ms, "partial_ms": t.partial_ms,
               "expected_ms": t.score_ms, "full_kernel_us": t.full_kernel_us,
               "partial_kernel_us": t.partial_kernel_us} for t in trials]
@@ -955,13 +962,14 @@ def main():
     edges = [e for e in rng.choice(2 * L - 2, size=64) if e != root]
     plists = [workloads.path_lists(tree, int(e)) for e in edges]
     ptimes = [workloads.edge_times(tree, pl[1]) for pl in plists]
-    for i in range(10):
+    for i in range(200):
         calc.propose_times(*plists[i % len(plists)], ptimes[i % len(plists)], rates)
         calc.total()
         calc.accept()
     # the rate unprofiled (MCMC's pattern: propose, read the total, accept),
-    # then the kernel time in a second, profiled pass over the same edges
-    nparts = max(50, args.steps)
+    # then the kernel time in a second, profiled pass over the same edges;
+    # 2,000 updates (about 0.1 s) so that one host hiccup does not set the rate
+    nparts = max(2000, args.steps)
 
     def partial_pass():
         t0 = time.perf_counter()
