@@ -7,9 +7,16 @@
   patterns, evaluated in one launch (single slot, as benchmarked); every
   instance's total against the plain FP64 GPU path, and a sample of
   instances (totals and per-site values) against the oracle itself.
+- GY94 codons on the matrix cores: 64 taxa, 20,000 unique simulated codon
+  patterns, Gamma4 (the codon benchmark workload), matrices built on the
+  device; per site, per category and in total against the oracle, then
+  one-edge partial evaluations after node-height moves, bitwise against a
+  fresh full evaluation of the moved tree.
 
 Tolerance (FP32 partials, FP64 root and sums): total relative <= 1e-6, per
-site |d lnL| <= 2e-6 max(1, |lnL|).
+site |d lnL| <= 2e-6 max(1, |lnL|).  Codons (FP16 matrices and partials):
+per site and category |d lnL| <= 3 (L - 1) 2^-11, the bound derived in
+test_gpu_codon.py, and total relative <= 1e-6.
 """
 import os
 
@@ -153,3 +160,69 @@ def test_batched_hky_gamma4_256_instances(hip_device):
         print(f"[full-shape] instance {i} ({x['L']} taxa, {x['P']} patterns) vs oracle: "
               f"{abs(totals[i] - ref) / abs(ref):.2e}", flush=True)
         del lik, ot
+
+
+def test_gy94_codon_64taxa_20000pat(hip_device):
+    wl = workloads.gy94_codon(64, 20_000)
+    L, P, K = 64, 20_000, wl["K"]
+    print(f"[full-shape] GY94 codons {L} taxa {P} patterns", flush=True)
+    model, rates = wl["model"], wl["rates"]
+    pi = model.frequencies()
+    tree = DatedTree(L, wl["parents"], wl["children"], wl["heights"])
+    calc = Calculator.HIPCodon(61).make(wl["codes"], P, K)
+    calc.set_model(model.q_matrix(), pi)
+    full = workloads.full_lists(tree)
+    calc.propose_times(*full, workloads.edge_times(tree, full[1]), rates)
+    calc.accept()
+    got = np.array([calc.likelihoods(k) for k in range(K)])
+    total, mixed = calc.total(), calc.site_mixed()
+
+    otree = O.Tree(L, wl["parents"], wl["children"], wl["heights"])
+    om = O.Model.reversible(model.exchangeability(), pi)
+    flat = np.asarray(full[1]).reshape(-1)
+    ncores = len(os.sched_getaffinity(0))
+    ref = []
+    for c in range(K):
+        mats = np.stack([om.transition(otree.edge_length(int(m)) * rates[c]) for m in flat])
+        cn = O.CalcN(wl["codes"], P, 61, 30, ncores)
+        cn.propose(full[0], full[1], mats, pi)
+        ref.append(cn.likelihoods())
+    ref = np.array(ref)
+    mx = ref.max(0)
+    ref_mixed = mx + np.log(np.exp(ref - mx).sum(0)) + np.log(1.0 / K)
+    bound = 3 * (L - 1) * 2.0 ** -11
+    err = np.abs(got - ref)
+    print(f"  per-category site max {err.max():.3e} (bound {bound:.3e}); total rel "
+          f"{abs(total - ref_mixed.sum()) / abs(ref_mixed.sum()):.2e}", flush=True)
+    assert err.max() <= bound
+    assert np.abs(mixed - ref_mixed).max() <= bound
+    assert abs(total - ref_mixed.sum()) <= 1e-6 * abs(ref_mixed.sum())
+
+    # one-edge partial evaluations, each accepted: bitwise equal to a fresh
+    # full evaluation of the moved tree (every node computes the same FP16
+    # values whether its children come from registers or from memory)
+    fresh = Calculator.HIPCodon(61).make(wl["codes"], P, K)
+    fresh.set_model(model.q_matrix(), pi)
+    rng = np.random.default_rng(9)
+    try:
+        for step in range(3):
+            node = int(rng.integers(L, 2 * L - 2))  # not the root
+            kids = tree.children_of(node)
+            lo = max(tree.heights[kids[0]], tree.heights[kids[1]])
+            hi = tree.heights[tree.parent_of(node)]
+            tree.set_height(node, lo + (hi - lo) * float(rng.uniform(0.05, 0.95)))
+            internals, children = tree.partials_lists()
+            calc.propose_times(internals, children, workloads.edge_times(tree, children),
+                               rates)
+            calc.accept()
+            tree.accept()
+            whole = workloads.full_lists(tree)
+            fresh.propose_times(*whole, workloads.edge_times(tree, whole[1]), rates)
+            fresh.accept()
+            for k in range(K):
+                np.testing.assert_array_equal(calc.likelihoods(k), fresh.likelihoods(k),
+                                              err_msg=f"step {step} category {k}")
+            assert calc.total() == fresh.total()
+    finally:
+        fresh.close()
+        calc.close()
