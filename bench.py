@@ -487,12 +487,29 @@ def bench_mcmc(args):
                  wl["freqs"], wl["alpha"], seed=101, tune_every=max(50, args.burnin // 40),
                  tune_cutoff=args.burnin, **kw)
     t0 = time.perf_counter()
+    thin = max(1, args.collect_every)
     collect = args.collect if 0 < args.collect < args.burnin else args.burnin
-    if collect < args.burnin:   # training pairs from the last `collect` steps only
-        burn.run(args.burnin - collect)
-    _, bstats, (X, y) = burn.run(collect, collect=collect)
+    block = 100
+    if thin > 1:
+        collect = min(collect, args.burnin // thin) // block * block
+    span = collect * thin  # the burn-in steps the training pairs come from
+    if span < args.burnin:   # training pairs from the last `span` steps only
+        burn.run(args.burnin - span)
+    if thin == 1:
+        _, bstats, (X, y) = burn.run(collect, collect=collect)
+    else:
+        # a chain at 200,000 sites drifts for tens of thousands of steps
+        # between independent states: pairs spread over a longer stretch
+        # cover more of what the measured chains visit
+        xs, ys = [], []
+        for _ in range(collect // block):
+            _, bstats, (xb, yb) = burn.run(block, collect=block)
+            xs.append(xb)
+            ys.append(yb)
+            _, bstats, _ = burn.run(block * (thin - 1))
+        X, y = np.concatenate(xs), np.concatenate(ys)
     log(f"[bench] burn-in {args.burnin} steps: {time.perf_counter() - t0:.1f} s, "
-        f"{len(y)} exact evaluations collected from the last {collect}")
+        f"{len(y)} exact evaluations collected from the last {span}")
     state = burn.params()
     nf = burn.num_features
     tuned = dict(scale_factor=bstats["tuning"][1], delta_tuning=bstats["tuning"][2],
@@ -598,7 +615,8 @@ def bench_mcmc(args):
                    "model": "GTR+Gamma4 alpha 0.5", "surrogate": f"{nf}->256->256->1 GELU BF16",
                    "burnin_steps": args.burnin, "chain_steps": args.mcmc_steps,
                    "trace_every": -(-args.mcmc_steps // 100_000),
-                   "training_pairs_from_last_steps": collect, "surrogate_betas": betas,
+                   "training_pairs_from_last_steps": span, "training_pairs": int(len(y)),
+                   "surrogate_betas": betas,
                    "chain_seed": args.chain_seed,
                    "topology_moves": "surrogate stage, then exact" if args.screen_topology
                                      else "exact MH test only",
@@ -890,6 +908,9 @@ def main():
     ap.add_argument("--clip", type=float, default=100.0)
     ap.add_argument("--collect", type=int, default=50_000,
                     help="surrogate training pairs from the last N burn-in steps (0: all)")
+    ap.add_argument("--collect-every", type=int, default=1,
+                    help="spread the training pairs over the last collect x N burn-in steps: "
+                         "blocks of 100 consecutive steps collected, then 100 (N - 1) not")
     ap.add_argument("--betas", default="0.5",
                     help="delayed acceptance: first-stage temperatures, one chain each; "
                          "the first is the headline")
