@@ -6,9 +6,11 @@
 // even, refreshed after every update) and accumulate in FP32.  Biases stay
 // FP32.  The loss is the mean squared error of the standardised output.
 //
-// The layers are small (404 x 256, 256 x 256), so every kernel is a plain
-// FP32-accumulating loop with the reused operand staged in LDS; a training
-// minibatch is a dozen launches.
+// The layers are small (404 x 256, 256 x 256 at a minibatch of 256): the
+// products are 32 x 32-tiled FP32 loops with both operands staged in LDS,
+// far below the sizes where matrix cores would pay for themselves against
+// launch latency.  A training minibatch is a dozen launches, enqueued
+// without a host synchronisation until the end of the epoch.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -27,7 +29,6 @@ b3h_status host_fail(b3h_status s, const std::string &msg);
 
 namespace {
 
-constexpr uint32_t kRows = 16;  // batch rows per forward workgroup
 
 __host__ __device__ inline uint16_t to_bf16(float f) {
   uint32_t u;
@@ -71,35 +72,6 @@ __global__ void k_gather_y(const float *yall, const uint32_t *idx, float ymean, 
   t[b] = (yall[idx ? idx[b] : b] - ymean) / ystd;
 }
 
-// Z[b][n] = sum_k A[b][k] W[k][n] + bias[n], Hout = gelu(Z).  Workgroup:
-// kRows batch rows x 256 columns; the rows' inputs are staged in LDS.
-__global__ __launch_bounds__(256) void k_dense_fwd(const float *A, const uint16_t *W,
-                                                   const float *bias, float *Z, float *Hout,
-                                                   uint32_t B, uint32_t K, uint32_t N) {
-  extern __shared__ float a_s[];  // [kRows][K]
-  const uint32_t b0 = blockIdx.y * kRows, n = blockIdx.x * 256u + threadIdx.x;
-  for (uint32_t i = threadIdx.x; i < kRows * K; i += 256u) {
-    const uint32_t r = i / K, k = i % K;
-    a_s[i] = b0 + r < B ? A[(size_t)(b0 + r) * K + k] : 0.0f;
-  }
-  __syncthreads();
-  if (n >= N) return;
-  float acc[kRows];
-#pragma unroll
-  for (uint32_t r = 0; r < kRows; r++) acc[r] = bias[n];
-  for (uint32_t k = 0; k < K; k++) {
-    const float w = from_bf16(W[(size_t)k * N + n]);
-#pragma unroll
-    for (uint32_t r = 0; r < kRows; r++) acc[r] = fmaf(a_s[r * K + k], w, acc[r]);
-  }
-#pragma unroll
-  for (uint32_t r = 0; r < kRows; r++) {
-    if (b0 + r >= B) break;
-    Z[(size_t)(b0 + r) * N + n] = acc[r];
-    Hout[(size_t)(b0 + r) * N + n] = gelu(acc[r]);
-  }
-}
-
 // y[b] = sum_k H[b][k] w[k] + bias: one wavefront per row
 __global__ __launch_bounds__(64) void k_out_fwd(const float *Hm, const uint16_t *w,
                                                 const float *bias, float *y, uint32_t K) {
@@ -110,76 +82,185 @@ __global__ __launch_bounds__(64) void k_out_fwd(const float *Hm, const uint16_t 
   if (l == 0) y[b] = s + bias[0];
 }
 
-// dY = 2 (y - t) / B; loss_acc += mean (y - t)^2 (one workgroup)
+// dY = 2 (y - t) / B; loss_acc += mean (y - t)^2; g_b3 = sum_b dY[b]
+// (one workgroup)
 __global__ __launch_bounds__(256) void k_loss(const float *y, const float *t, float *dy,
-                                              float *loss_acc, uint32_t B) {
-  __shared__ float part[256];
-  float s = 0.0f;
+                                              float *loss_acc, float *g_b3, uint32_t B) {
+  __shared__ float part[256], gpart[256];
+  float s = 0.0f, gs = 0.0f;
   for (uint32_t b = threadIdx.x; b < B; b += 256u) {
     const float e = y[b] - t[b];
-    dy[b] = 2.0f * e / (float)B;
+    const float d = 2.0f * e / (float)B;
+    dy[b] = d;
     s += e * e;
+    gs += d;
   }
   part[threadIdx.x] = s;
+  gpart[threadIdx.x] = gs;
   __syncthreads();
   for (uint32_t o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    if (threadIdx.x < o) {
+      part[threadIdx.x] += part[threadIdx.x + o];
+      gpart[threadIdx.x] += gpart[threadIdx.x + o];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) loss_acc[0] += part[0] / (float)B;
+  if (threadIdx.x == 0) {
+    loss_acc[0] += part[0] / (float)B;
+    g_b3[0] = gpart[0];
+  }
 }
 
-// output layer backward: g_w3[k] = sum_b dy[b] H2[b][k] (and g_b3), and
-// dZ2[b][k] = dy[b] w3[k] gelu'(Z2[b][k])
-__global__ __launch_bounds__(256) void k_out_bwd(const float *dy, const float *H2,
-                                                 const float *Z2, const uint16_t *w3,
-                                                 float *g_w3, float *g_b3, float *dZ2, uint32_t B,
-                                                 uint32_t K) {
-  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
-  if (k < K) {
-    float g = 0.0f;
-    const float w = from_bf16(w3[k]);
-    for (uint32_t b = 0; b < B; b++) {
-      g = fmaf(dy[b], H2[(size_t)b * K + k], g);
-      dZ2[(size_t)b * K + k] = dy[b] * w * gelu_grad(Z2[(size_t)b * K + k]);
+// Products of the layers: C[M][N] = op(A)[M][K] op(B)[K][N] with FP32
+// accumulation, A FP32 and B FP32 or BF16 (weights), either transposed in
+// memory:
+//   op(A)[i][k] = TA ? A[k lda + i] : A[i lda + k]
+//   op(B)[k][j] = TB ? B[j ldb + k] : B[k ldb + j]
+// Workgroup: a 32 x 32 tile of C and four wavefronts, each summing a quarter
+// of every 64-deep chunk of the reduction (16 steps) into 4 x 4 outputs per
+// lane; the four partial tiles are added in wavefront order at the end.  The
+// chunks pass through LDS, each operand loaded with consecutive threads on
+// consecutive addresses whichever way it is stored, and the next chunk's
+// loads are in flight while the current one is summed.  At 256 x 404 x 256
+// that is 64 workgroups of four busy SIMDs each; a split by output alone
+// leaves one wavefront per SIMD waiting on every LDS read.  Epilogue:
+//   kEpiBias      C = acc + bias[j], C2 = gelu(C)    (forward layer)
+//   kEpiStore     C = acc                            (weight gradient)
+//   kEpiGeluGrad  C = acc gelu'(Zp[i][j])            (input gradient)
+constexpr int kEpiBias = 0, kEpiStore = 1, kEpiGeluGrad = 2;
+constexpr uint32_t kTile = 32, kKT = 64, kLd = kTile + 4;
+constexpr uint32_t kPer = kTile * kKT / 256;  // operand elements per thread and chunk
+// B entries are held as loaded (FP32 bits, or a zero-extended BF16) and
+// widened when stored to LDS: a conversion next to its conditional load
+// would make every load wait for the one before it
+__device__ __forceinline__ uint32_t ld_raw(const float *p, size_t i) { return __float_as_uint(p[i]); }
+__device__ __forceinline__ uint32_t ld_raw(const uint16_t *p, size_t i) { return p[i]; }
+__device__ __forceinline__ float widen(const float *, uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ float widen(const uint16_t *, uint32_t u) {
+  return __uint_as_float(u << 16);
+}
+template <bool TA, bool TB, typename TBt, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(const float *A, uint32_t lda, const TBt *Bm,
+                                              uint32_t ldb, uint32_t M, uint32_t N, uint32_t K,
+                                              float *C, float *C2, const float *bias,
+                                              const float *Zp) {
+  // [k][i] and [k][j] chunks (rows padded to 36 floats: 16-byte aligned
+  // reads of four consecutive entries); afterwards the four partial tiles
+  __shared__ __align__(16) float sm[2 * kKT * kLd];
+  float *As = sm, *Bs = sm + kKT * kLd;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t tx = lane % 8u, ty = lane / 8u;
+  const uint32_t i0 = blockIdx.y * kTile, j0 = blockIdx.x * kTile;
+  // chunk element e of each operand: (i, k) of A and (k, j) of B, the
+  // contiguous index of the stored layout varying fastest
+  auto a_ik = [](uint32_t e, uint32_t &i, uint32_t &k) {
+    i = TA ? e % kTile : e / kKT;
+    k = TA ? e / kTile : e % kKT;
+  };
+  auto b_kj = [](uint32_t e, uint32_t &k, uint32_t &j) {
+    k = TB ? e % kKT : e / kTile;
+    j = TB ? e / kKT : e % kTile;
+  };
+  float ra[kPer];
+  uint32_t rb[kPer];
+  auto fetch = [&](uint32_t k0) {
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; r++) {
+      const uint32_t e = tid + 256u * r;
+      uint32_t i, k, j, kb;
+      a_ik(e, i, k);
+      b_kj(e, kb, j);
+      ra[r] = (i0 + i < M && k0 + k < K)
+                  ? (TA ? A[(size_t)(k0 + k) * lda + i0 + i] : A[(size_t)(i0 + i) * lda + k0 + k])
+                  : 0.0f;
+      rb[r] = (k0 + kb < K && j0 + j < N)
+                  ? (TB ? ld_raw(Bm, (size_t)(j0 + j) * ldb + k0 + kb)
+                        : ld_raw(Bm, (size_t)(k0 + kb) * ldb + j0 + j))
+                  : 0u;
     }
-    g_w3[k] = g;
+  };
+  float acc[4][4] = {};
+  fetch(0);
+  for (uint32_t k0 = 0; k0 < K; k0 += kKT) {
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; r++) {
+      const uint32_t e = tid + 256u * r;
+      uint32_t i, k, j, kb;
+      a_ik(e, i, k);
+      b_kj(e, kb, j);
+      As[k * kLd + i] = ra[r];
+      Bs[kb * kLd + j] = widen(Bm, rb[r]);
+    }
+    __syncthreads();
+    if (k0 + kKT < K) fetch(k0 + kKT);
+#pragma unroll 4
+    for (uint32_t kk = 16u * w; kk < 16u * w + 16u; kk++) {
+      const float4 a = *reinterpret_cast<const float4 *>(As + kk * kLd + 4u * ty);
+      const float4 b = *reinterpret_cast<const float4 *>(Bs + kk * kLd + 4u * tx);
+      const float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (uint32_t r = 0; r < 4; r++)
+#pragma unroll
+        for (uint32_t q = 0; q < 4; q++) acc[r][q] = fmaf(av[r], bv[q], acc[r][q]);
+    }
+    __syncthreads();
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    float s = 0.0f;
-    for (uint32_t b = 0; b < B; b++) s += dy[b];
-    g_b3[0] = s;
+  // partial tiles [w][32][32], then each thread finishes four outputs
+  float *red = sm;
+#pragma unroll
+  for (uint32_t r = 0; r < 4; r++)
+#pragma unroll
+    for (uint32_t q = 0; q < 4; q++)
+      red[(w * kTile + 4u * ty + r) * kTile + 4u * tx + q] = acc[r][q];
+  __syncthreads();
+#pragma unroll
+  for (uint32_t u = 0; u < 4; u++) {
+    const uint32_t o_t = tid + 256u * u, ti = o_t / kTile, tj = o_t % kTile;
+    const uint32_t i = i0 + ti, j = j0 + tj;
+    if (i >= M || j >= N) continue;
+    float v = red[ti * kTile + tj];
+#pragma unroll
+    for (uint32_t q = 1; q < 4; q++) v += red[(q * kTile + ti) * kTile + tj];
+    const size_t o = (size_t)i * N + j;
+    if (EPI == kEpiBias) {
+      const float z = v + bias[j];
+      C[o] = z;
+      C2[o] = gelu(z);
+    } else if (EPI == kEpiStore) {
+      C[o] = v;
+    } else {
+      C[o] = v * gelu_grad(Zp[o]);
+    }
   }
 }
 
-// G[k][n] = sum_b A[b][k] D[b][n]; workgroup = one k, 256 columns
-__global__ __launch_bounds__(256) void k_wgrad(const float *A, const float *Dm, float *G,
-                                               uint32_t B, uint32_t K, uint32_t N) {
-  const uint32_t k = blockIdx.y, n = blockIdx.x * 256u + threadIdx.x;
-  if (n >= N) return;
-  float g = 0.0f;
-  for (uint32_t b = 0; b < B; b++) g = fmaf(A[(size_t)b * K + k], Dm[(size_t)b * N + n], g);
-  G[(size_t)k * N + n] = g;
-}
-
-__global__ __launch_bounds__(256) void k_bgrad(const float *Dm, float *g, uint32_t B, uint32_t N) {
-  const uint32_t n = blockIdx.x * 256u + threadIdx.x;
-  if (n >= N) return;
+// g[n] = sum_b s[b] D[b][n] (s null: 1), in a fixed order: 16 wavefronts
+// take every 16th row, then one lane per column adds the 16 sums in order.
+__global__ __launch_bounds__(1024) void k_colsum(const float *Dm, const float *scale, float *g,
+                                                 uint32_t B, uint32_t N) {
+  __shared__ float part[16][64];
+  const uint32_t l = threadIdx.x & 63u, w = threadIdx.x >> 6, n = blockIdx.x * 64u + l;
   float s = 0.0f;
-  for (uint32_t b = 0; b < B; b++) s += Dm[(size_t)b * N + n];
-  g[n] = s;
+  if (n < N)
+    for (uint32_t b = w; b < B; b += 16u) s = fmaf(scale ? scale[b] : 1.0f, Dm[(size_t)b * N + n], s);
+  part[w][l] = s;
+  __syncthreads();
+  if (w == 0 && n < N) {
+    float t = 0.0f;
+    for (uint32_t q = 0; q < 16u; q++) t += part[q][l];
+    g[n] = t;
+  }
 }
 
-// dZp[b][k] = (sum_n D[b][n] W[k][n]) gelu'(Zp[b][k])
-__global__ __launch_bounds__(256) void k_igrad(const float *Dm, const uint16_t *W,
-                                               const float *Zp, float *dZp, uint32_t B, uint32_t K,
-                                               uint32_t N) {
+// dZ2[b][k] = dy[b] w3[k] gelu'(Z2[b][k])
+__global__ __launch_bounds__(256) void k_out_bwd(const float *dy, const float *Z2,
+                                                 const uint16_t *w3, float *dZ2, uint32_t B,
+                                                 uint32_t K) {
   const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
-  if (i >= (size_t)B * K) return;
-  const uint32_t b = (uint32_t)(i / K), k = (uint32_t)(i % K);
-  float s = 0.0f;
-  for (uint32_t n = 0; n < N; n++) s = fmaf(Dm[(size_t)b * N + n], from_bf16(W[(size_t)k * N + n]), s);
-  dZp[i] = s * gelu_grad(Zp[i]);
+  if (i < (size_t)B * K) {
+    const uint32_t b = (uint32_t)(i / K), k = (uint32_t)(i % K);
+    dZ2[i] = dy[b] * from_bf16(w3[k]) * gelu_grad(Z2[i]);
+  }
 }
 
 // Adam (bias-corrected) on the FP32 masters, then the BF16 copy
@@ -197,24 +278,26 @@ __global__ void k_adam(float *p, const float *g, float *m, float *v, uint16_t *p
   pb[i] = to_bf16(q);
 }
 
-// One sample, one workgroup: the whole forward pass with the input passed by
-// value (no copy to the device) and the result written to pinned host
-// memory (no copy back): one launch per surrogate call in the MCMC loop.
-// The hidden units use the same FP32 summation order as k_dense_fwd.  The
-// weights are read from transposed copies (one padded row of BF16 per
-// hidden unit, k_transpose), so a lane loads eight weights per 16-byte load
-// and all of a row's loads are independent: in the [k][n] layout of the
-// training kernels a lane read two bytes per load, and the pass was bound
-// by one CU's memory latency (about 40 us per call).
+// One sample: the whole forward pass in one launch, with the input passed by
+// value (no copy to the device) and the result written to pinned host memory
+// (no copy back): one launch per surrogate call in the MCMC loop.  The
+// weights are read from transposed copies (one zero-padded row of BF16 per
+// hidden unit, k_transpose), eight weights per 16-byte load.  The pass is
+// spread over ceil(H / 16) workgroups: workgroup g computes first-layer
+// units [16 g, 16 g + 16) (16 lanes per unit, each summing every 16th group
+// of eight inputs, then a butterfly over the 16 lanes) and those units'
+// contribution to every second-layer unit; the workgroup that takes the
+// last ticket adds the contributions in workgroup order and finishes the
+// output layer.  (In one workgroup the pass was bound by one CU's load
+// bandwidth for the 0.3 MB of weights: 12 us per call.)
 constexpr uint32_t kMaxFusedD = 512, kMaxFusedH = 1024;
+constexpr uint32_t kUnits = 16;  // first-layer units per workgroup
 struct FusedInput {
   float x[kMaxFusedD];
 };
-// row of weights (BF16, zero-padded from `used` to a multiple of 8) dotted
-// with v in index order, starting from z: whole groups of eight weights,
-// sixteen 16-byte loads in flight at a time, then the last partial group
 __device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+// eight BF16 weights dotted with v in index order, starting from z
 __device__ __forceinline__ float dot8(const uint4 w, const float *v, float z) {
   z = fmaf(v[0], bf16_lo(w.x), z);
   z = fmaf(v[1], bf16_hi(w.x), z);
@@ -225,55 +308,88 @@ __device__ __forceinline__ float dot8(const uint4 w, const float *v, float z) {
   z = fmaf(v[6], bf16_lo(w.w), z);
   return fmaf(v[7], bf16_hi(w.w), z);
 }
-__device__ __forceinline__ float dot_row(const uint16_t *row, const float *v, uint32_t used,
-                                         float z) {
-  const uint4 *r = reinterpret_cast<const uint4 *>(row);
-  const uint32_t full = used / 8u;
-  uint32_t g = 0;
-  constexpr uint32_t kBatch = 16;  // 16-byte loads in flight per lane
-  for (; g + kBatch <= full; g += kBatch) {
-    uint4 w[kBatch];
-#pragma unroll
-    for (uint32_t q = 0; q < kBatch; q++) w[q] = r[g + q];
-#pragma unroll
-    for (uint32_t q = 0; q < kBatch; q++) z = dot8(w[q], v + 8u * (g + q), z);
-  }
-  for (; g < full; g++) z = dot8(r[g], v + 8u * g, z);
-  if (used % 8u) {
-    const uint4 w = r[full];
-    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-    for (uint32_t j = 0; j < used % 8u; j++)
-      z = fmaf(v[8u * full + j], (j & 1u) ? bf16_hi(ws[j / 2]) : bf16_lo(ws[j / 2]), z);
-  }
-  return z;
+__device__ __forceinline__ void wait_stores() {
+  __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));  // vmcnt(0)
 }
 
+// grid ceil(H / kUnits) x 256 threads; `in` holds the standardised input
+// (the host applies the input scaling, so no load of it is on the critical
+// path).  part [grid][H] holds the second-layer contributions; ticket is zero
+// between launches (the last workgroup resets it).  Every row access stays
+// inside its zero-padded row: Dp and Hp are multiples of 8 and each
+// workgroup's unit range starts at a multiple of 16 below H.  The
+// second-layer weights a thread needs first are loaded at the start, so that
+// their latency overlaps the first layer's.
 __global__ __launch_bounds__(256) void k_predict1(FusedInput in, uint32_t D, uint32_t Dp,
-                                                  uint32_t H, uint32_t Hp, const float *xmean,
-                                                  const float *xstd, const uint16_t *W1t,
+                                                  uint32_t H, uint32_t Hp, const uint16_t *W1t,
                                                   const float *b1, const uint16_t *W2t,
                                                   const float *b2, const uint16_t *w3,
                                                   const float *b3, float ymean, float ystd,
-                                                  float *out, uint32_t *flag, uint32_t seq) {
-  __shared__ float xs[kMaxFusedD];
-  __shared__ float h1[kMaxFusedH];
-  __shared__ float part[4];
-  const uint32_t tid = threadIdx.x;
-  for (uint32_t k = tid; k < D; k += 256u) xs[k] = (in.x[k] - xmean[k]) / xstd[k];
+                                                  float *part, uint32_t *ticket, float *out,
+                                                  uint32_t *flag, uint32_t seq) {
+  __shared__ __align__(16) float xs[kMaxFusedD];
+  __shared__ __align__(16) float h1[kUnits];
+  __shared__ float red[4];
+  __shared__ int last;
+  const uint32_t tid = threadIdx.x, g = blockIdx.x, G = gridDim.x;
+  const uint32_t m0 = g * kUnits;
+  const bool two = m0 + 8u < Hp;
+  uint4 wa = make_uint4(0u, 0u, 0u, 0u), wb = wa;
+  if (tid < H) {
+    const uint4 *row = reinterpret_cast<const uint4 *>(W2t + (size_t)tid * Hp + m0);
+    wa = row[0];
+    if (two) wb = row[1];
+  }
+  for (uint32_t k = tid; k < Dp; k += 256u) xs[k] = k < D ? in.x[k] : 0.0f;
   __syncthreads();
-  for (uint32_t n = tid; n < H; n += 256u)
-    h1[n] = gelu(dot_row(W1t + (size_t)n * Dp, xs, D, b1[n]));
+  // first layer: unit u of this workgroup, lane j of its 16
+  const uint32_t u = tid >> 4, j = tid & 15u, n1 = m0 + u;
+  float z = 0.0f;
+  if (n1 < H) {
+    const uint4 *row = reinterpret_cast<const uint4 *>(W1t + (size_t)n1 * Dp);
+    for (uint32_t q = j; q < Dp / 8u; q += 16u) z = dot8(row[q], xs + 8u * q, z);
+  }
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) z += __shfl_xor(z, o, 16);
+  if (j == 0) h1[u] = n1 < H ? gelu(z + b1[n1]) : 0.0f;
   __syncthreads();
+  // these units' contribution to every second-layer unit
+  for (uint32_t n = tid; n < H; n += 256u) {
+    if (n != tid) {
+      const uint4 *row = reinterpret_cast<const uint4 *>(W2t + (size_t)n * Hp + m0);
+      wa = row[0];
+      if (two) wb = row[1];
+    }
+    float s = dot8(wa, h1, 0.0f);
+    if (two) s = dot8(wb, h1 + 8, s);
+    __hip_atomic_store(&part[(size_t)g * H + n], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // the ticket only after every contribution is stored (order, no flush:
+  // the agent-scope stores write to the level every XCD reads)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  wait_stores();
+  __syncthreads();
+  if (tid == 0)
+    last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+           G - 1u;
+  __syncthreads();
+  if (!last) return;
+  wait_stores();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   float s = 0.0f;
   for (uint32_t n = tid; n < H; n += 256u) {
-    const float z = dot_row(W2t + (size_t)n * Hp, h1, H, b2[n]);
-    s = fmaf(gelu(z), from_bf16(w3[n]), s);
+    float z2 = b2[n];
+    for (uint32_t q = 0; q < G; q++)
+      z2 += __hip_atomic_load(&part[(size_t)q * H + n], __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+    s = fmaf(gelu(z2), from_bf16(w3[n]), s);
   }
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if ((tid & 63u) == 0) part[tid >> 6] = s;
+  if ((tid & 63u) == 0) red[tid >> 6] = s;
   __syncthreads();
   if (tid == 0) {
-    out[0] = (part[0] + part[1] + part[2] + part[3] + b3[0]) * ystd + ymean;
+    out[0] = (red[0] + red[1] + red[2] + red[3] + b3[0]) * ystd + ymean;
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // fine-grained pinned memory: a host that sees seq also sees the value
     __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
@@ -307,12 +423,14 @@ struct b3m_mlp {
   uint32_t Dp = 0, Hp = 0;
   bool t_stale = true;
   float *xmean = nullptr, *xstd = nullptr;
+  std::vector<float> hmean, hstd;  // host copies: the one-sample pass scales on the host
   float ymean = 0.0f, ystd = 1.0f;
   uint32_t bcap = 0;
   float *X = nullptr, *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr, *H2 = nullptr, *Y = nullptr,
         *T = nullptr, *dY = nullptr, *dZ1 = nullptr, *dZ2 = nullptr, *loss = nullptr;
   float *xin = nullptr;
-  uint32_t *idx = nullptr;
+  float *part = nullptr;       // [ceil(H / 16)][H] second-layer contributions
+  uint32_t *ticket = nullptr;  // last-workgroup ticket of the one-sample pass
   float *h_out = nullptr;  // pinned result of the fused single-sample pass
   uint32_t *h_flag = nullptr;  // sequence number stored after the result
   uint32_t seq = 0;
@@ -357,8 +475,6 @@ b3h_status ensure_batch(b3m_mlp *m, uint32_t cap) {
   }
   b3h_status st = malloc_dev(m, (void **)&m->xin, (size_t)cap * m->D * 4);
   if (st) return st;
-  st = malloc_dev(m, (void **)&m->idx, (size_t)cap * 4);
-  if (st) return st;
   m->bcap = cap;
   return B3H_OK;
 }
@@ -367,11 +483,11 @@ b3h_status ensure_batch(b3m_mlp *m, uint32_t cap) {
 hipError_t forward(b3m_mlp *m, uint32_t B) {
   const uint32_t D = m->D, H = m->H;
   const uint16_t *W1 = m->pb + m->oW1, *W2 = m->pb + m->oW2, *w3 = m->pb + m->ow3;
-  dim3 g1((H + 255) / 256, (B + kRows - 1) / kRows);
-  hipLaunchKernelGGL(k_dense_fwd, g1, dim3(256), kRows * D * 4, m->stream, m->X, W1,
-                     m->p + m->ob1, m->Z1, m->H1, B, D, H);
-  hipLaunchKernelGGL(k_dense_fwd, g1, dim3(256), kRows * H * 4, m->stream, m->H1, W2,
-                     m->p + m->ob2, m->Z2, m->H2, B, H, H);
+  const dim3 g((H + kTile - 1) / kTile, (B + kTile - 1) / kTile);
+  hipLaunchKernelGGL((k_gemm<false, false, uint16_t, kEpiBias>), g, dim3(256), 0, m->stream,
+                     m->X, D, W1, H, B, H, D, m->Z1, m->H1, m->p + m->ob1, nullptr);
+  hipLaunchKernelGGL((k_gemm<false, false, uint16_t, kEpiBias>), g, dim3(256), 0, m->stream,
+                     m->H1, H, W2, H, B, H, H, m->Z2, m->H2, m->p + m->ob2, nullptr);
   hipLaunchKernelGGL(k_out_fwd, dim3(B), dim3(64), 0, m->stream, m->H2, w3, m->p + m->ob3, m->Y,
                      H);
   return hipGetLastError();
@@ -380,19 +496,31 @@ hipError_t forward(b3m_mlp *m, uint32_t B) {
 // loss and gradients (into m->g) for targets m->T
 hipError_t backward(b3m_mlp *m, uint32_t B) {
   const uint32_t D = m->D, H = m->H;
-  hipLaunchKernelGGL(k_loss, dim3(1), dim3(256), 0, m->stream, m->Y, m->T, m->dY, m->loss, B);
-  hipLaunchKernelGGL(k_out_bwd, dim3((H + 255) / 256), dim3(256), 0, m->stream, m->dY, m->H2,
-                     m->Z2, m->pb + m->ow3, m->g + m->ow3, m->g + m->ob3, m->dZ2, B, H);
-  hipLaunchKernelGGL(k_wgrad, dim3((H + 255) / 256, H), dim3(256), 0, m->stream, m->H1, m->dZ2,
-                     m->g + m->oW2, B, H, H);
-  hipLaunchKernelGGL(k_bgrad, dim3((H + 255) / 256), dim3(256), 0, m->stream, m->dZ2,
-                     m->g + m->ob2, B, H);
-  hipLaunchKernelGGL(k_igrad, dim3((B * H + 255) / 256), dim3(256), 0, m->stream, m->dZ2,
-                     m->pb + m->oW2, m->Z1, m->dZ1, B, H, H);
-  hipLaunchKernelGGL(k_wgrad, dim3((H + 255) / 256, D), dim3(256), 0, m->stream, m->X, m->dZ1,
-                     m->g + m->oW1, B, D, H);
-  hipLaunchKernelGGL(k_bgrad, dim3((H + 255) / 256), dim3(256), 0, m->stream, m->dZ1,
-                     m->g + m->ob1, B, H);
+  const uint16_t *W2 = m->pb + m->oW2;
+  const dim3 cs((H + 63) / 64);
+  hipLaunchKernelGGL(k_loss, dim3(1), dim3(256), 0, m->stream, m->Y, m->T, m->dY, m->loss,
+                     m->g + m->ob3, B);
+  // output layer: dZ2, then g_w3[k] = sum_b dy[b] H2[b][k]
+  hipLaunchKernelGGL(k_out_bwd, dim3((unsigned)(((size_t)B * H + 255) / 256)), dim3(256), 0,
+                     m->stream, m->dY, m->Z2, m->pb + m->ow3, m->dZ2, B, H);
+  hipLaunchKernelGGL(k_colsum, cs, dim3(1024), 0, m->stream, m->H2, m->dY, m->g + m->ow3, B, H);
+  // second layer: g_W2 = H1^T dZ2, g_b2, dZ1 = (dZ2 W2^T) gelu'(Z1)
+  hipLaunchKernelGGL((k_gemm<true, false, float, kEpiStore>),
+                     dim3((H + kTile - 1) / kTile, (H + kTile - 1) / kTile), dim3(256), 0,
+                     m->stream, m->H1, H, m->dZ2, H, H, H, B, m->g + m->oW2, nullptr, nullptr,
+                     nullptr);
+  hipLaunchKernelGGL(k_colsum, cs, dim3(1024), 0, m->stream, m->dZ2, nullptr, m->g + m->ob2, B,
+                     H);
+  hipLaunchKernelGGL((k_gemm<false, true, uint16_t, kEpiGeluGrad>),
+                     dim3((H + kTile - 1) / kTile, (B + kTile - 1) / kTile), dim3(256), 0,
+                     m->stream, m->dZ2, H, W2, H, B, H, H, m->dZ1, nullptr, nullptr, m->Z1);
+  // first layer: g_W1 = X^T dZ1, g_b1
+  hipLaunchKernelGGL((k_gemm<true, false, float, kEpiStore>),
+                     dim3((H + kTile - 1) / kTile, (D + kTile - 1) / kTile), dim3(256), 0,
+                     m->stream, m->X, D, m->dZ1, H, D, H, B, m->g + m->oW1, nullptr, nullptr,
+                     nullptr);
+  hipLaunchKernelGGL(k_colsum, cs, dim3(1024), 0, m->stream, m->dZ1, nullptr, m->g + m->ob1, B,
+                     H);
   return hipGetLastError();
 }
 
@@ -411,12 +539,13 @@ b3h_status mlp_predict(b3m_mlp *m, const float *x, uint32_t B, float *y) {
       MCHK(hipGetLastError());
       m->t_stale = false;
     }
+    // (x - mean) / std in FP32, as k_gather computes it on the device
     FusedInput in;
-    std::memcpy(in.x, x, (size_t)m->D * 4);
-    hipLaunchKernelGGL(k_predict1, dim3(1), dim3(256), 0, m->stream, in, m->D, m->Dp, m->H,
-                       m->Hp, m->xmean, m->xstd, m->w1t, m->p + m->ob1, m->w2t, m->p + m->ob2,
-                       m->pb + m->ow3, m->p + m->ob3, m->ymean, m->ystd, m->h_out, m->h_flag,
-                       ++m->seq);
+    for (uint32_t k = 0; k < m->D; k++) in.x[k] = (x[k] - m->hmean[k]) / m->hstd[k];
+    hipLaunchKernelGGL(k_predict1, dim3((m->H + kUnits - 1) / kUnits), dim3(256), 0, m->stream,
+                       in, m->D, m->Dp, m->H, m->Hp, m->w1t, m->p + m->ob1,
+                       m->w2t, m->p + m->ob2, m->pb + m->ow3, m->p + m->ob3, m->ymean, m->ystd,
+                       m->part, m->ticket, m->h_out, m->h_flag, ++m->seq);
     MCHK(hipGetLastError());
     // poll for the launch's sequence number (up to 20 ms), else wait for
     // the stream, which also reports any error of the launch
@@ -496,7 +625,10 @@ b3h_status b3m_mlp_create(int device, uint32_t inputs, uint32_t hidden, uint64_t
       (st = malloc_dev(m, (void **)&m->w2t, (size_t)hidden * ((hidden + 7u) & ~7u) * 2)) ||
       (st = malloc_dev(m, (void **)&m->xmean, (size_t)inputs * 4)) ||
       (st = malloc_dev(m, (void **)&m->xstd, (size_t)inputs * 4)) ||
-      (st = malloc_dev(m, (void **)&m->loss, 16)))
+      (st = malloc_dev(m, (void **)&m->loss, 16)) ||
+      (st = malloc_dev(m, (void **)&m->part,
+                       (size_t)((hidden + kUnits - 1) / kUnits) * hidden * 4)) ||
+      (st = malloc_dev(m, (void **)&m->ticket, 64)))
     return fail(st);
   // He-uniform initial weights (GELU), zero biases, identity scaling
   std::vector<float> p(m->np, 0.0f);
@@ -509,9 +641,12 @@ b3h_status b3m_mlp_create(int device, uint32_t inputs, uint32_t hidden, uint64_t
   fill(m->oW2, (size_t)hidden * hidden, hidden);
   fill(m->ow3, hidden, hidden);
   std::vector<float> zeros(inputs, 0.0f), ones(inputs, 1.0f);
+  m->hmean = zeros;
+  m->hstd = ones;
   bool ok = hipMemcpy(m->p, p.data(), m->np * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemset(m->m, 0, m->np * 4) == hipSuccess &&
             hipMemset(m->v, 0, m->np * 4) == hipSuccess &&
+            hipMemset(m->ticket, 0, 64) == hipSuccess &&
             hipMemcpy(m->xmean, zeros.data(), inputs * 4, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(m->xstd, ones.data(), inputs * 4, hipMemcpyHostToDevice) == hipSuccess;
   if (ok) {
@@ -551,6 +686,8 @@ b3h_status b3m_mlp_set_scaling(b3m_mlp *m, const float *x_mean, const float *x_s
   (void)hipSetDevice(m->device);
   MCHK(hipMemcpy(m->xmean, x_mean, (size_t)m->D * 4, hipMemcpyHostToDevice));
   MCHK(hipMemcpy(m->xstd, x_std, (size_t)m->D * 4, hipMemcpyHostToDevice));
+  m->hmean.assign(x_mean, x_mean + m->D);
+  m->hstd.assign(x_std, x_std + m->D);
   m->ymean = y_mean;
   m->ystd = y_std;
   return B3H_OK;
@@ -615,16 +752,20 @@ b3h_status b3m_mlp_train(b3m_mlp *m, const float *x, const float *y, uint32_t n,
   b3h_status st = ensure_batch(m, batch);
   if (st) return st;
   float *xall = nullptr, *yall = nullptr;
+  uint32_t *dorder = nullptr;
   if (hipMalloc((void **)&xall, (size_t)n * m->D * 4) != hipSuccess ||
-      hipMalloc((void **)&yall, (size_t)n * 4) != hipSuccess) {
+      hipMalloc((void **)&yall, (size_t)n * 4) != hipSuccess ||
+      hipMalloc((void **)&dorder, (size_t)n * 4) != hipSuccess) {
     (void)hipGetLastError();
     if (xall) (void)hipFree(xall);
+    if (yall) (void)hipFree(yall);
     return b3h::host_fail(B3H_ERR_OOM, "hipMalloc failed for the training set");
   }
   auto done = [&](b3h_status s) {
     (void)hipStreamSynchronize(m->stream);
     (void)hipFree(xall);
     (void)hipFree(yall);
+    (void)hipFree(dorder);
     return s;
   };
   if (hipMemcpy(xall, x, (size_t)n * m->D * 4, hipMemcpyHostToDevice) != hipSuccess ||
@@ -636,18 +777,21 @@ b3h_status b3m_mlp_train(b3m_mlp *m, const float *x, const float *y, uint32_t n,
   const uint32_t nb = n / batch;  // full minibatches only
   float last = 0.0f;
   for (uint32_t ep = 0; ep < epochs; ep++) {
+    // the stream is idle here (the previous epoch ended with a
+    // synchronisation), so the blocking upload of this epoch's order is
+    // complete before its first minibatch reads it
     std::shuffle(order.begin(), order.end(), rng);
+    if (hipMemcpy(dorder, order.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return done(b3h::host_fail(B3H_ERR_HIP, "index upload failed"));
     if (hipMemsetAsync(m->loss, 0, 4, m->stream) != hipSuccess)
       return done(b3h::host_fail(B3H_ERR_HIP, "hipMemsetAsync failed"));
     for (uint32_t bi = 0; bi < nb; bi++) {
-      if (hipMemcpyAsync(m->idx, order.data() + (size_t)bi * batch, (size_t)batch * 4,
-                         hipMemcpyHostToDevice, m->stream) != hipSuccess)
-        return done(b3h::host_fail(B3H_ERR_HIP, "index upload failed"));
+      const uint32_t *idx = dorder + (size_t)bi * batch;
       const size_t tot = (size_t)batch * m->D;
       hipLaunchKernelGGL(k_gather, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, m->stream,
-                         xall, m->idx, m->xmean, m->xstd, m->X, batch, m->D);
+                         xall, idx, m->xmean, m->xstd, m->X, batch, m->D);
       hipLaunchKernelGGL(k_gather_y, dim3((batch + 255) / 256), dim3(256), 0, m->stream, yall,
-                         m->idx, m->ymean, m->ystd, m->T, batch);
+                         idx, m->ymean, m->ystd, m->T, batch);
       if (forward(m, batch) != hipSuccess || backward(m, batch) != hipSuccess)
         return done(b3h::host_fail(B3H_ERR_HIP, "surrogate training launch failed"));
       m->adam_t++;
@@ -657,13 +801,13 @@ b3h_status b3m_mlp_train(b3m_mlp *m, const float *x, const float *y, uint32_t n,
       hipLaunchKernelGGL(k_adam, dim3((unsigned)((m->np + 255) / 256)), dim3(256), 0, m->stream,
                          m->p, m->g, m->m, m->v, m->pb, m->np, lr, b1, b2, 1e-8f, c1, c2);
       m->t_stale = true;
-      // synchronise before the next index upload reuses the host order buffer
-      if (hipStreamSynchronize(m->stream) != hipSuccess)
-        return done(b3h::host_fail(B3H_ERR_HIP, "surrogate training failed"));
+      if (hipGetLastError() != hipSuccess)
+        return done(b3h::host_fail(B3H_ERR_HIP, "surrogate training launch failed"));
     }
     float acc = 0.0f;
-    if (hipMemcpy(&acc, m->loss, 4, hipMemcpyDeviceToHost) != hipSuccess)
-      return done(b3h::host_fail(B3H_ERR_HIP, "loss readback failed"));
+    if (hipMemcpyAsync(&acc, m->loss, 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+      return done(b3h::host_fail(B3H_ERR_HIP, "surrogate training failed"));
     last = acc / (float)std::max(nb, 1u);
   }
   if (final_loss) *final_loss = last;
