@@ -68,7 +68,11 @@ b3h_status b3m_mlp_destroy(b3m_mlp *m);
 b3h_status b3m_mlp_set_scaling(b3m_mlp *m, const float *x_mean, const float *x_std,
                                float y_mean, float y_std);
 
-/* y[batch] from x[batch][inputs] (host arrays) */
+/* y[batch] from x[batch][inputs] (host arrays).  A batch of one (with
+ * inputs <= 512 and hidden <= 1024) is one launch whose result is polled
+ * from pinned host memory; it sums in another order than larger batches,
+ * so the two agree to FP32 rounding.  Calls on one handle must not overlap:
+ * they share the handle's device buffers. */
 b3h_status b3m_mlp_forward(b3m_mlp *m, const float *x, uint32_t batch, float *y);
 
 /* Mean squared error of the standardised output and its gradient with
