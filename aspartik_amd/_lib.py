@@ -50,7 +50,8 @@ class Config(C.Structure):
         ("scale_ln", C.c_uint32),
         ("tile_patterns", C.c_uint32),
         ("lane_patterns", C.c_uint32),
-        ("reserved", C.c_uint32 * 2),
+        ("lane_states", C.c_uint32),
+        ("reserved", C.c_uint32 * 1),
     ]
 
 

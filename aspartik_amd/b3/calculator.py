@@ -19,18 +19,25 @@ class Calculator:
     """Backend choice + scaling, like b3's ``CalculatorConfig``."""
 
     def __init__(self, precision: int, device: int = 0, tile_patterns: int = 0,
-                 scale_ln: int = 30, states: int = 4, lane_patterns: int = 0):
+                 scale_ln: int = 30, states: int = 4, lane_patterns: int = 0,
+                 lane_states: int = 0):
         self.precision = int(precision)
         self.device = int(device)
         self.tile_patterns = int(tile_patterns)
         self.lane_patterns = int(lane_patterns)
+        self.lane_states = int(lane_states)
         self.scale_ln = int(scale_ln)
         self.states = int(states)
 
     @classmethod
-    def HIP(cls, device: int = 0, tile_patterns: int = 0, lane_patterns: int = 0):
-        """FP32 partials with exact power-of-two scaling (the product path)."""
-        return cls(_lib.B3H_F32_SCALED, device, tile_patterns, lane_patterns=lane_patterns)
+    def HIP(cls, device: int = 0, tile_patterns: int = 0, lane_patterns: int = 0,
+            lane_states: int = 0):
+        """FP32 partials with exact power-of-two scaling (the product path).
+        tile_patterns / 64: wavefronts per category in a workgroup;
+        lane_patterns: patterns per lane (1, 2; 0 = by size); lane_states: 4
+        (default) or 1, four lanes per pattern for small pattern counts."""
+        return cls(_lib.B3H_F32_SCALED, device, tile_patterns, lane_patterns=lane_patterns,
+                   lane_states=lane_states)
 
     @classmethod
     def HIPPlainF64(cls, device: int = 0):
@@ -103,6 +110,7 @@ class HipCalculator:
         cfg.scale_ln = config.scale_ln
         cfg.tile_patterns = config.tile_patterns
         cfg.lane_patterns = config.lane_patterns
+        cfg.lane_states = config.lane_states
         h = C.c_void_p()
         check(lib().b3h_create(C.byref(cfg), samples, self.L, self.P, self.K, C.byref(h)))
         self._h = h

@@ -51,14 +51,17 @@ def _propose(calc, lst, rates):
 
 
 def default_candidates(num_categories: int, device: int = 0):
-    """Partial storage with every tile that fits a 1024-thread workgroup and
-    1 or 2 patterns per lane, plus propagation storage."""
+    """Partial storage with every tile that fits a 1024-thread workgroup: four
+    states per lane with 1 or 2 patterns per lane, and one state per lane
+    (four lanes per pattern); plus propagation storage."""
     out = []
     t = 1
     while 64 * num_categories * t <= 1024:
         for ppl in (1, 2):
             out.append((f"partials T={t} ppl={ppl}",
                         Calculator.HIP(device, tile_patterns=64 * t, lane_patterns=ppl)))
+        out.append((f"partials T={t} states/lane=1",
+                    Calculator.HIP(device, tile_patterns=64 * t, lane_states=1)))
         t *= 2
     out.append(("propagations", Calculator.HIPPropagations(device)))
     return out

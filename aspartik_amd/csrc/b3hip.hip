@@ -58,6 +58,7 @@ struct b3h_calc {
   uint32_t scale_ln = 30;
   uint32_t T = 1;    // 64-lane pattern rows (wavefronts per category) per workgroup
   uint32_t PPL = 1;  // patterns per lane
+  bool S1 = false;   // full evaluations with one state per lane (16 patterns per wavefront)
   uint32_t cstride = 0;  // device code row stride (P rounded up to 64)
   uint32_t blocks = 0;
 
@@ -526,10 +527,16 @@ b3h_status run_propose(b3h_calc *c, uint32_t n, const uint32_t *internals,
     } else {
       // stored cherries are recomputed only by the HBM-prefetch variant
       const bool hbm = c->sched.hbm_sources * 8 > n || c->sched.cherry_sources > 0;
-      const uint32_t ppl = hbm ? 1u : c->PPL;
-      const uint32_t tp = 64u * c->T * ppl;
-      a.ntiles = (c->P + tp - 1) / tp;
-      HIPCHK(launch_prune_f32(a, ppl, hbm, (c->P + tp - 1) / tp, c->stream));  // hbm => ppl 1
+      if (c->S1 && !hbm) {
+        const uint32_t tp = 16u * c->T;
+        a.ntiles = (c->P + tp - 1) / tp;
+        HIPCHK(launch_prune_f32_s1(a, a.ntiles, c->stream));
+      } else {
+        const uint32_t ppl = hbm ? 1u : c->PPL;
+        const uint32_t tp = 64u * c->T * ppl;
+        a.ntiles = (c->P + tp - 1) / tp;
+        HIPCHK(launch_prune_f32(a, ppl, hbm, a.ntiles, c->stream));  // hbm => ppl 1
+      }
     }
     if (e1) HIPCHK(hipEventRecord(e1, c->stream));
   } else {
@@ -840,8 +847,16 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
     // Tile choice: T wavefront rows per category (64*K*T <= 1024 threads) and
     // PPL patterns per lane.  Two patterns per lane once the wavefront count
     // (K*P/64) exceeds what the chip holds at once (~16 per CU x 256 CUs).
+    const uint32_t SPL = cfg->lane_states ? cfg->lane_states : 4u;
+    if (SPL != 4 && SPL != 1) {
+      fail(B3H_ERR_ARG, "lane_states must be 0, 1 or 4");
+      return bail(B3H_ERR_ARG);
+    }
     uint32_t PPL = cfg->lane_patterns;
+    if (SPL == 1) PPL = 1;  // the four-states forms it falls back to take one pattern per lane
     if (PPL == 0) PPL = ((uint64_t)K * P / 64 > 4096) ? 2 : 1;
+    // patterns one wavefront row of a tile covers in full evaluations
+    const uint32_t row = SPL == 1 ? 16u : 64u * PPL;
     uint32_t T = 1;
     if (cfg->tile_patterns) {
       T = std::max<uint32_t>(1u, cfg->tile_patterns / 64u);
@@ -853,7 +868,7 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
       // T = 4 takes 106 us, T = 1 79 us)
       constexpr uint32_t kMinTiles = 192;
       T = std::max<uint32_t>(1u, 16u / K);
-      while (T > 1 && (P + 64u * T * PPL - 1) / (64u * T * PPL) < kMinTiles) T--;
+      while (T > 1 && (P + row * T - 1) / (row * T) < kMinTiles) T--;
     }
     while (64u * K * T > 1024u && T > 1) T--;
     if (PPL > 2) {
@@ -862,12 +877,20 @@ b3h_status b3h_create(const b3h_config *cfg, const uint8_t *samples,
     }
     c->T = T;
     c->PPL = PPL;  // upper bound; partial updates launch with one pattern per lane
-    const uint32_t TP = 64 * T * PPL;
-    c->blocks = (P + 64 * T - 1) / (64 * T);  // the most blocks any launch uses
-    // LDS stack capacity for this tile shape and the longest update list
+    c->S1 = SPL == 1;
+    const uint32_t TP = 64 * T * PPL;  // a multiple of the one-state tile, 16 T
+    // the most blocks any launch uses
+    c->blocks = c->S1 ? (P + 16 * T - 1) / (16 * T) : (P + 64 * T - 1) / (64 * T);
+    // LDS stack capacity for this tile shape (and, with one state per lane,
+    // the four-states form partial updates fall back to) and the longest
+    // update list
+    auto lds = [&](uint32_t cap) {
+      const size_t b = prune_f32_lds_bytes(K, T, cap, c->I, PPL);
+      return c->S1 ? std::max(b, prune_f32_s1_lds_bytes(K, T, cap, c->I)) : b;
+    };
     uint32_t cap = kMaxSlots;
-    while (cap > 1 && prune_f32_lds_bytes(K, T, cap, c->I, PPL) > kMaxLds) cap--;
-    if (prune_f32_lds_bytes(K, T, cap, c->I, PPL) > kMaxLds) {
+    while (cap > 1 && lds(cap) > kMaxLds) cap--;
+    if (lds(cap) > kMaxLds) {
       fail(B3H_ERR_UNSUPPORTED, "tree too large for the LDS-staged update list");
       return bail(B3H_ERR_UNSUPPORTED);
     }

@@ -82,6 +82,11 @@ hipError_t launch_prune_f32(const F32Args &a, uint32_t ppl, bool hbm, uint32_t b
                             hipStream_t s);
 size_t prune_f32_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n,
                            uint32_t ppl);
+// One state per lane (a quad per pattern, 16 patterns per wavefront, 16 T
+// per tile): lists whose stored children are read in place only (full
+// evaluations); values bit-identical to launch_prune_f32's.
+hipError_t launch_prune_f32_s1(const F32Args &a, uint32_t blocks, hipStream_t s);
+size_t prune_f32_s1_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n);
 // persistent batched form over many instances (one F32Args each)
 hipError_t prepare_prune_f32_batch(int max_lds_bytes);
 hipError_t batch_grid(uint32_t ppl, bool hbm, uint32_t threads, size_t lds, uint32_t *grid);

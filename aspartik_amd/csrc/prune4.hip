@@ -181,6 +181,19 @@ size_t prune_f32_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n,
   return b;
 }
 
+size_t prune_f32_s1_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n) {
+  const size_t W = (size_t)K * T;
+  const size_t TP = 16 * (size_t)T;
+  size_t b = (size_t)n * 16;
+  b += 2 * ((size_t)CH * 2 * K * 80);
+  b += 2 * ((size_t)CH * 2 * TP);
+  b = (b + 15) & ~(size_t)15;
+  const size_t stacks = W * nslots * 64 * (4 + 2);
+  const size_t mixbuf = W * 16 * 8 + T * 8 + 16;
+  b += stacks > mixbuf ? stacks : mixbuf;
+  return b;
+}
+
 // HBM: the update list reads children from HBM (partial updates); those are
 // then fetched one node ahead.  Without it (full evaluations) the rare HBM
 // child (LDS stack overflow) is loaded where it is used, and the prefetch
@@ -539,6 +552,268 @@ __device__ __forceinline__ void prune_tile(A &a, const uint32_t tile, unsigned c
 #undef B3H_FETCH_HBM
 }
 
+// One state per lane.  A quad of lanes holds one pattern's four states, so a
+// wavefront covers 16 patterns and a tile four times as many wavefronts as
+// the four-states-per-lane form: the shape for pattern counts too small to
+// fill the chip, where a tile's walk through the update list is bound by
+// latency rather than by issue or HBM.  Lane s of a quad keeps state s; a
+// projection reads row s of the matrix and the child's four states from the
+// quad (DPP broadcasts), in the order apply() uses, and the normalisation
+// takes the quad's maximum, so every partial, exponent and site value is
+// bit for bit what the other shapes compute.  Lists with stored children in
+// HBM beyond the rare in-place load (partial updates, stored cherries) keep
+// the four-states-per-lane forms (host rule).
+template <int CTRL>
+__device__ __forceinline__ float quad_dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+constexpr int kQuadB0 = 0x00, kQuadB1 = 0x55, kQuadB2 = 0xAA, kQuadB3 = 0xFF;  // broadcast lane k
+constexpr int kQuadSwap1 = 0xB1, kQuadSwap2 = 0x4E;  // [1,0,3,2], [2,3,0,1]
+
+template <typename A>
+__device__ __forceinline__ void prune_tile_s1(A &a, const uint32_t tile, unsigned char *smem) {
+  constexpr uint32_t MAT_REGS = 4;   // matrix-chunk dwords per thread
+  constexpr uint32_t CODE_REGS = 1;  // code-chunk dwords per thread
+  const uint32_t nthr = blockDim.x;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const uint32_t w = uni(tid >> 6);
+  const uint32_t K = a.K, T = a.T, n = a.n;
+  const uint32_t W = K * T;
+  const uint32_t c = w % K, t = w / K;
+  const uint32_t TP = 16u * T;  // patterns per tile
+  const uint32_t tile0 = tile * TP;
+  const uint32_t s = lane & 3u, pl = lane >> 2;  // state, pattern of the wavefront
+  const uint32_t pt = t * 16u + pl;              // pattern of the tile
+  const uint32_t p = tile0 + pt;
+  const bool active = p < a.P;
+  const uint32_t pp = active ? p : 0u;
+  // byte offsets of this lane's value in a partial row and of its pattern's
+  // exponent in an exponent row
+  const uint32_t poff = p * 16u + s * 4u, eoff = p * 2u;
+  const uint32_t ns = a.nslots;
+  const size_t KP = (size_t)K * a.P;
+  const uint32_t MATC = CH * 2 * K * 20;  // floats per matrix chunk
+  const uint32_t CODEC = CH * 2 * TP;     // bytes per code chunk
+
+  // ---- LDS carve-up (mirrors prune_f32_s1_lds_bytes) ----
+  unsigned char *sp = smem;
+  uint4 *ent_s = reinterpret_cast<uint4 *>(sp);
+  sp += (size_t)n * 16;
+  float *mat_s = reinterpret_cast<float *>(sp);  // [2][MATC]
+  sp += 2 * (size_t)MATC * 4;
+  unsigned char *code_s = sp;  // [2][CODEC]
+  sp += 2 * (size_t)CODEC;
+  sp = smem + ((sp - smem + 15) & ~(size_t)15);
+  float *stk_all = reinterpret_cast<float *>(sp);  // [W][ns][64]
+  int16_t *stke_all = reinterpret_cast<int16_t *>(stk_all + (size_t)W * ns * 64);
+  double *mix = reinterpret_cast<double *>(sp);  // [W][16], after the node loop only
+  double *red = mix + (size_t)W * 16;
+  int *flag = reinterpret_cast<int *>(red + T);
+  float *stk = stk_all + (size_t)w * ns * 64;
+  int16_t *stke = stke_all + (size_t)w * ns * 64;
+
+  const uint4 *g_ent = reinterpret_cast<const uint4 *>(a.entries);
+  for (uint32_t i = tid; i < n; i += nthr) ent_s[i] = g_ent[i];
+  __syncthreads();
+
+  // chunk fetch into registers / store into an LDS buffer, as prune_tile
+  float mreg[MAT_REGS];
+  uint32_t creg[CODE_REGS];
+  const uint32_t TPW = TP / 4;
+  auto fetch = [&](uint32_t ck) {
+    const uint32_t base = ck * CH;
+    const uint32_t cnt = min(CH, n - base);
+    const uint32_t nm = cnt * 2 * K * 20;
+    const float *gm = a.pmats + (size_t)base * 2 * K * 20;
+#pragma unroll
+    for (uint32_t r = 0; r < MAT_REGS; r++) {
+      const uint32_t q = tid + r * nthr;
+      mreg[r] = q < nm ? gm[q] : 0.0f;
+    }
+    const uint32_t nc = cnt * 2 * TPW;
+#pragma unroll
+    for (uint32_t r = 0; r < CODE_REGS; r++) {
+      const uint32_t q = tid + r * nthr;
+      uint32_t v = 0x04040404u;
+      if (q < nc) {
+        const uint32_t jj = q / (2 * TPW);
+        const uint32_t side = (q / TPW) & 1u;
+        const uint32_t k = q % TPW;
+        const uint4 e = ent_s[base + jj];
+        const uint32_t src = side ? e.z : e.y;
+        if ((src & SRC_TAG) == SRC_TIP)
+          v = a.codes32[(size_t)(src & SRC_IDX) * (a.cstride / 4) + tile0 / 4 + k];
+      }
+      creg[r] = v;
+    }
+  };
+  auto stash = [&](uint32_t buf, uint32_t ck) {
+    const uint32_t base = ck * CH;
+    const uint32_t cnt = min(CH, n - base);
+    const uint32_t nm = cnt * 2 * K * 20;
+    const uint32_t nc = cnt * 2 * TPW;
+    float *ms = mat_s + (size_t)buf * MATC;
+    uint32_t *cs = reinterpret_cast<uint32_t *>(code_s + (size_t)buf * CODEC);
+#pragma unroll
+    for (uint32_t r = 0; r < MAT_REGS; r++) {
+      const uint32_t q = tid + r * nthr;
+      if (q < nm) ms[q] = mreg[r];
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < CODE_REGS; r++) {
+      const uint32_t q = tid + r * nthr;
+      if (q < nc) cs[q] = creg[r];
+    }
+  };
+
+  const uint32_t nchunks = (n + CH - 1) / CH;
+  fetch(0u);
+  stash(0u, 0u);
+  __syncthreads();
+
+  float x = 0.0f;
+  int E = 0;
+  for (uint32_t ck = 0; ck < nchunks; ck++) {
+    const uint32_t buf = ck & 1u;
+    const bool more = ck + 1 < nchunks;
+    if (more) fetch(ck + 1);
+    const uint32_t base = ck * CH;
+    const uint32_t cnt = min(CH, n - base);
+    const float *ms = mat_s + (size_t)buf * MATC;
+    const unsigned char *cs = code_s + (size_t)buf * CODEC;
+    for (uint32_t j = 0; j < cnt; j++) {
+      const uint32_t i = base + j;
+      const uint4 e4 = ((ConstU4 *)a.entries)[i];
+      const uint32_t out = uni(e4.x), lsrc = uni(e4.y), rsrc = uni(e4.z), meta = uni(e4.w);
+      // one child: a tip is entry `code` of row s (column `code`, component
+      // s; the fifth column, all ones, for an unknown code); any other
+      // child is row s dotted with the child's four states
+      auto child = [&](uint32_t src, uint32_t side, float &v, int &ee) {
+        const float *mc = ms + ((size_t)(j * 2 + side) * K + c) * 20;
+        const uint32_t tag = src & SRC_TAG, idx = src & SRC_IDX;
+        if (tag == SRC_TIP) {
+          v = mc[(uint32_t)cs[(j * 2 + side) * TP + pt] * 4u + s];
+          ee = 0;
+          return;
+        }
+        const float m0 = mc[s], m1 = mc[4 + s], m2 = mc[8 + s], m3 = mc[12 + s];
+        float y;
+        if (tag == SRC_PREV) {
+          y = x;
+          ee = E;
+        } else if (tag == SRC_LDS) {
+          y = stk[idx * 64 + lane];
+          ee = exp_unpack(stke[idx * 64 + lane]);
+        } else {
+          const size_t o = (size_t)(idx & SRC_SLOT) * KP + (size_t)c * a.P + pp;
+          y = reinterpret_cast<const float *>(a.partials)[o * 4 + s];
+          ee = exp_unpack(a.exps[o]);
+        }
+        const float y0 = quad_dpp<kQuadB0>(y), y1 = quad_dpp<kQuadB1>(y);
+        const float y2 = quad_dpp<kQuadB2>(y), y3 = quad_dpp<kQuadB3>(y);
+        v = fmaf(m3, y3, fmaf(m2, y2, fmaf(m1, y1, m0 * y0)));
+      };
+      float lv, rv;
+      int le, re;
+      child(lsrc, 0, lv, le);
+      child(rsrc, 1, rv, re);
+      float v = lv * rv;
+      float mx = fmaxf(v, quad_dpp<kQuadSwap1>(v));
+      mx = fmaxf(mx, quad_dpp<kQuadSwap2>(mx));
+      const int ex = __builtin_amdgcn_frexp_expf(mx);
+      v = __builtin_amdgcn_ldexpf(v, -ex);
+      x = v;
+      E = exp_clamp(le + re + ex);
+      const int16_t ep = exp_pack(E);
+      const uint32_t push = meta & 0xFFu;
+      const bool store = !(a.cmats && (lsrc & SRC_TAG) == SRC_TIP &&
+                           (rsrc & SRC_TAG) == SRC_TIP) || (meta & META_KEEP);
+      if (active && store) {
+        const size_t ob = (size_t)out * KP + (size_t)c * a.P;
+        float *dp = reinterpret_cast<float *>(reinterpret_cast<char *>(a.partials + ob) + poff);
+#if B3H_NT_STORE
+        __builtin_nontemporal_store(v, dp);
+        if (s == 0)
+          __builtin_nontemporal_store(ep, reinterpret_cast<int16_t *>(
+                                              reinterpret_cast<char *>(a.exps + ob) + eoff));
+#else
+        *dp = v;
+        if (s == 0) *reinterpret_cast<int16_t *>(reinterpret_cast<char *>(a.exps + ob) + eoff) = ep;
+#endif
+      }
+      if (push != NO_PUSH) {
+        stk[push * 64 + lane] = v;
+        stke[push * 64 + lane] = ep;
+      }
+    }
+    if (more) stash(buf ^ 1u, ck + 1);
+    __syncthreads();
+  }
+
+  // root, category mix and weighted sum as prune_tile, one pattern per quad
+  double pi0 = a.pi0, pi1 = a.pi1, pi2 = a.pi2, pi3 = a.pi3;
+  if (a.pik) {
+    pi0 = a.pik[4 * c];
+    pi1 = a.pik[4 * c + 1];
+    pi2 = a.pik[4 * c + 2];
+    pi3 = a.pik[4 * c + 3];
+  }
+  const float x0 = quad_dpp<kQuadB0>(x), x1 = quad_dpp<kQuadB1>(x);
+  const float x2 = quad_dpp<kQuadB2>(x), x3 = quad_dpp<kQuadB3>(x);
+  double site = -INFINITY;
+  if (active) {
+    double v = (double)x0 * pi0;
+    v = fma((double)x1, pi1, v);
+    v = fma((double)x2, pi2, v);
+    v = fma((double)x3, pi3, v);
+    site = exp_site(v, E);
+    if (s == 0) a.site[(size_t)c * a.P + p] = site;
+  }
+  if (s == 0) mix[w * 16u + pl] = site;
+  __syncthreads();
+  double wv = 0.0;
+  if (c == 0) {
+    if (s == 0 && active) {
+      double mixed;
+      if (a.classes) {
+        mixed = mix[(t * K + (uint32_t)a.classes[p]) * 16u + pl];
+      } else {
+        double mxv = -INFINITY;
+        for (uint32_t k = 0; k < K; k++) mxv = fmax(mxv, mix[(t * K + k) * 16u + pl]);
+        mixed = -INFINITY;
+        if (mxv != -INFINITY) {
+          double se = 0.0;
+          for (uint32_t k = 0; k < K; k++) se = se + exp(mix[(t * K + k) * 16u + pl] - mxv);
+          mixed = (mxv + log(se)) + a.ln_inv_k;
+        }
+      }
+      a.mixed[p] = mixed;
+      wv = mixed * (double)a.weights[p];
+    }
+    wv = wave_sum(wv);
+    if (lane == 0) red[t] = wv;
+  }
+  __syncthreads();
+  double bsum = 0.0;
+  if (tid == 0)
+    for (uint32_t tt = 0; tt < T; tt++) bsum += red[tt];
+  last_block_total(bsum, a.blocksum, a.counter, a.total, a.host_total, a.host_flag, a.seq, flag,
+                   tile, a.ntiles);
+}
+
+__global__ __launch_bounds__(1024) void prune_f32_s1_kernel(F32Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+#if defined(__HIP_DEVICE_COMPILE__)
+  (void)a;
+  typedef const __attribute__((address_space(4))) F32Args ConstArgs;
+  ConstArgs &ka = *(ConstArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+  prune_tile_s1(ka, blockIdx.x, smem);
+#else
+  prune_tile_s1(a, blockIdx.x, smem);
+#endif
+}
+
 template <int PPL, bool HBM>
 __global__ __launch_bounds__(1024) void prune_f32_kernel(F32Args a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -812,6 +1087,7 @@ hipError_t prepare_prune_f32(int max_lds_bytes) {
   const void *fns[] = {reinterpret_cast<const void *>(&prune_f32_kernel<1, false>),
                        reinterpret_cast<const void *>(&prune_f32_kernel<1, true>),
                        reinterpret_cast<const void *>(&prune_f32_kernel<2, false>),
+                       reinterpret_cast<const void *>(&prune_f32_s1_kernel),
                        reinterpret_cast<const void *>(&prune_path_kernel)};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -860,6 +1136,13 @@ hipError_t launch_prune_f32_batch(const F32Args *insts, const uint2 *items, uint
     hipLaunchKernelGGL((prune_f32_batch_kernel<1, true>), g, b, l, s, insts, items, nitems, queue);
   else
     hipLaunchKernelGGL((prune_f32_batch_kernel<1, false>), g, b, l, s, insts, items, nitems, queue);
+  return hipGetLastError();
+}
+
+hipError_t launch_prune_f32_s1(const F32Args &a, uint32_t blocks, hipStream_t s) {
+  const size_t lds = prune_f32_s1_lds_bytes(a.K, a.T, a.nslots, a.n);
+  hipLaunchKernelGGL(prune_f32_s1_kernel, dim3(blocks), dim3(64u * a.K * a.T), (uint32_t)lds, s,
+                     a);
   return hipGetLastError();
 }
 

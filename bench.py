@@ -895,6 +895,10 @@ def main():
                          "timed steps run at steady clocks")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-f32", action="store_true", help="profiling runs: FP32 kernel only")
+    ap.add_argument("--tile-patterns", type=int, default=0,
+                    help="FP32 calculator: wavefronts per category x 64 (0: by size)")
+    ap.add_argument("--lane-states", type=int, default=0, choices=[0, 1, 4],
+                    help="FP32 calculator: states per lane (0: 4)")
     ap.add_argument("--workload", choices=["gtr128", "c1", "batch", "codon", "mcmc", "hetero",
                                            "storage", "ingest"],
                     default="gtr128")
@@ -964,7 +968,8 @@ def main():
     workload_name = f"gtr_gamma4_{L}taxa_{P}pat"
 
     log("[bench] FP32 calculator")
-    calc = Calculator.HIP().make(wl["codes"], P, K)
+    calc = Calculator.HIP(tile_patterns=args.tile_patterns,
+                          lane_states=args.lane_states).make(wl["codes"], P, K)
     calc.set_weights(wl["weights"])
     calc.set_model(model.q_matrix(), model.frequencies(), True)
     dt, total32, kernel_mean_ms, each = time_evals(calc, full, times_full, rates, args.steps,
@@ -1116,7 +1121,8 @@ def main():
         "data": "synthetic (seeded Yule tree, sequences simulated along it)",
         "config": {"workload": workload_name, "taxa": L, "patterns": P, "categories": K,
                    "model": "GTR+Gamma4 alpha 0.5", "evaluation": "full (all edges dirty)",
-                   "seed": wl["seed"]},
+                   "seed": wl["seed"], "lane_states": args.lane_states or 4,
+                   "tile_patterns": args.tile_patterns or "by size"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,

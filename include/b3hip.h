@@ -72,7 +72,12 @@ typedef struct b3h_config {
                               only check it is nonzero.                       */
   uint32_t tile_patterns;  /* 64-pattern rows per workgroup x 64; 0 = tuned  */
   uint32_t lane_patterns;  /* patterns per lane (1 or 2); 0 = tuned           */
-  uint32_t reserved[2];
+  uint32_t lane_states;    /* FP32 partials: states per lane, 4 (a lane holds
+                              a pattern's four states) or 1 (four lanes per
+                              pattern, 16 patterns per wavefront: for pattern
+                              counts too small to fill the chip); 0 = 4.
+                              Values are bit-identical either way.           */
+  uint32_t reserved[1];
 } b3h_config;
 
 typedef struct b3h_calc b3h_calc;

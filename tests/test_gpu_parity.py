@@ -261,6 +261,70 @@ def test_tile_shapes_are_bit_identical(hip_device, tile, lanes):
     assert base.total() == t1
 
 
+@pytest.mark.parametrize("L,P,K,tile", [(33, 5003, 2, 64), (33, 5003, 2, 512), (33, 4001, 4, 256),
+                                         (33, 777, 1, 64), (400, 3000, 4, 256)])
+def test_one_state_per_lane_is_bit_identical(hip_device, L, P, K, tile):
+    """lane_states = 1 (four lanes per pattern, the quad exchanging states by
+    DPP) computes the same site values as four states per lane, bit for bit:
+    in full evaluations, in the partial updates that follow (which read the
+    partials it stored, through the four-states kernels), with per-pattern
+    classes, and with the in-place reads of stored children that a deep
+    update list needs once the LDS stack is full (400 taxa).  Totals agree to
+    the rounding of the weighted sum's grouping (16 patterns per wavefront
+    instead of 64 or 128)."""
+    rng, tree_arrays, om, pm, codes, w = _setup(L, P, K, seed=L + P)
+    rates = gamma_rates(0.5, K) if K > 1 else np.ones(1)
+
+    def make(**kw):
+        c = Calculator.HIP(**kw).make(codes, P, K)
+        c.set_weights(w)
+        c.set_model(pm.q_matrix(), pm.frequencies(), True)
+        return c
+
+    base, one = make(), make(tile_patterns=tile, lane_states=1)
+    tree = DatedTree(L, *tree_arrays)
+    tree.mark_all_edges_updated()
+
+    def evaluate(c, t):
+        internals, children = t.partials_lists()
+        flat = children.reshape(-1).astype(np.int64)
+        c.propose_times(internals, children, t.heights[t.parents[flat]] - t.heights[flat], rates)
+
+    def same():
+        for c in range(K):
+            np.testing.assert_array_equal(one.likelihoods(c), base.likelihoods(c))
+        np.testing.assert_array_equal(one.site_mixed(), base.site_mixed())
+        assert abs(one.total() - base.total()) <= 1e-13 * abs(base.total())
+
+    for c in (base, one):
+        evaluate(c, tree.copy())
+        c.accept()
+    tree.accept()
+    same()
+    # height moves: partial updates of the stored state, then a rejected one
+    for step in range(4):
+        cand = [nd for nd in range(L, 2 * L - 1) if nd != tree.root]
+        nd = int(rng.choice(cand))
+        lo = max(tree.heights[ch] for ch in tree.children_of(nd))
+        tree.set_height(nd, float(lo + (tree.heights[tree.parent_of(nd)] - lo) * 0.3))
+        for c in (base, one):
+            evaluate(c, tree.copy())
+        same()
+        for c in (base, one):
+            c.accept() if step < 3 else c.reject()
+        tree.accept() if step < 3 else tree.reject()
+    # per-pattern classes (HeteroLikelihood) over a fresh full evaluation
+    classes = rng.integers(0, K, size=P).astype(np.uint32)
+    for c in (base, one):
+        c.set_classes(classes)
+        t2 = tree.copy()
+        t2.mark_all_edges_updated()
+        evaluate(c, t2)
+    same()
+    base.close()
+    one.close()
+
+
 def test_protocol_errors(hip_device):
     L, P, K = 6, 10, 1
     rng, tree, om, pm, codes, w = _setup(L, P, K, seed=1)
