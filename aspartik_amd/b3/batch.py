@@ -14,8 +14,12 @@ from .._lib import check, lib
 
 class Batch:
     def __init__(self, instances, num_categories: int = 4, device: int = 0,
-                 single_slot: bool = False, tile_patterns: int = 0, lane_patterns: int = 0):
-        """instances: sequence of (codes [L, P] u8, weights [P] u32 or None)."""
+                 single_slot: bool = False, tile_patterns: int = 0, lane_patterns: int = 0,
+                 lane_states: int = 0):
+        """instances: sequence of (codes [L, P] u8, weights [P] u32 or None).
+        Tile shape: tile_patterns / 64 wavefronts per category, lane_patterns
+        patterns per lane (1 or 2; 0 = 2), lane_states states per lane (4, or
+        1: four lanes per pattern; 0 = 4)."""
         self._keep = []
         arr = (_lib.Instance * len(instances))()
         for i, (codes, weights) in enumerate(instances):
@@ -39,6 +43,7 @@ class Batch:
         cfg.device = device
         cfg.tile_patterns = tile_patterns
         cfg.lane_patterns = lane_patterns
+        cfg.lane_states = lane_states
         h = C.c_void_p()
         flags = _lib.B3H_BATCH_SINGLE_SLOT if single_slot else 0
         check(lib().b3h_batch_create(C.byref(cfg), self.count, arr, self.K, flags, C.byref(h)))

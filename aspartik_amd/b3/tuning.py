@@ -136,18 +136,20 @@ def autotune(codes, weights, tree: DatedTree, substitution, category_rates,
 class BatchTrial:
     tile_patterns: int   # patterns per lane row of a workgroup's tile (64 x sub-tiles)
     lane_patterns: int   # patterns per lane
+    lane_states: int     # states per lane (4, or 1: four lanes per pattern)
     ms: float            # per batch evaluation: captured-graph replay, synchronous
     kernel_ms: float     # the batched pruning kernel (HIP events)
 
 
 def batch_candidates(num_categories: int):
-    """Tile shapes of the batched kernel (workgroups of at most 512 threads):
-    every power-of-two number of 64-pattern sub-tiles, one or two patterns
-    per lane."""
+    """Tile shapes of the batched kernel (workgroups of at most 512 threads)
+    as (tile_patterns, lane_patterns, lane_states): every power-of-two
+    number of wavefronts per category, four states per lane with one or two
+    patterns per lane, and one state per lane."""
     t = max(1, 512 // (64 * num_categories))
     out = []
     while t >= 1:
-        out += [(64 * t, 1), (64 * t, 2)]
+        out += [(64 * t, 1, 4), (64 * t, 2, 4), (64 * t, 1, 1)]
         t //= 2
     return out
 
@@ -155,18 +157,18 @@ def batch_candidates(num_categories: int):
 def autotune_batch(instances, models, updates, num_categories: int, evals: int = 8,
                    candidates=None, single_slot: bool = True, device: int = 0):
     """Measured tile choice for one batch: each candidate (tile_patterns,
-    lane_patterns) evaluates the caller's own instances -- one eager
-    evaluation, then ``evals`` captured-graph replays -- and the fastest
-    wins.  instances: [(codes, weights)]; models: [(q, frequencies,
+    lane_patterns, lane_states) evaluates the caller's own instances -- one
+    eager evaluation, then ``evals`` captured-graph replays -- and the
+    fastest wins.  instances: [(codes, weights)]; models: [(q, frequencies,
     special_cases)] per instance; updates: one update per instance, as for
-    ``Batch.propose_times``.  Returns ((tile_patterns, lane_patterns),
-    trials sorted fastest first).  Every shape computes the same per-site
-    values; only the schedule differs."""
+    ``Batch.propose_times``.  Returns ((tile_patterns, lane_patterns,
+    lane_states), trials sorted fastest first).  Every shape computes the
+    same per-site values; only the schedule differs."""
     from .batch import Batch
     trials = []
-    for tp, lp in (candidates or batch_candidates(num_categories)):
+    for tp, lp, ls in (candidates or batch_candidates(num_categories)):
         b = Batch(instances, num_categories, device, single_slot, tile_patterns=tp,
-                  lane_patterns=lp)
+                  lane_patterns=lp, lane_states=ls)
         try:
             for i, (q, f, special) in enumerate(models):
                 b.set_model(i, q, f, special)
@@ -190,6 +192,7 @@ def autotune_batch(instances, models, updates, num_categories: int, evals: int =
             b.profile_enable(False)
         finally:
             b.close()
-        trials.append(BatchTrial(tp, lp, ms, kms / max(1, kn)))
+        trials.append(BatchTrial(tp, lp, ls, ms, kms / max(1, kn)))
     trials.sort(key=lambda tr: tr.ms)
-    return (trials[0].tile_patterns, trials[0].lane_patterns), trials
+    best = trials[0]
+    return (best.tile_patterns, best.lane_patterns, best.lane_states), trials

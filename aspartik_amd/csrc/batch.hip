@@ -65,6 +65,7 @@ struct b3h_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   uint32_t count = 0, K = 0, S = 4, T = 1, PPL = 2;
+  bool S1 = false;  // full evaluations with one state per lane (items of 16 T patterns)
   bool single = false;
   std::vector<BInst> inst;
   uint32_t cap = 8;
@@ -135,8 +136,8 @@ void batch_release(b3h_batch *b) {
   delete b;
 }
 
-uint32_t tiles_for(const b3h_batch *b, const BInst &in, uint32_t ppl) {
-  const uint32_t tp = 64u * b->T * ppl;
+uint32_t tiles_for(const b3h_batch *b, const BInst &in, uint32_t ppl, bool s1) {
+  const uint32_t tp = s1 ? 16u * b->T : 64u * b->T * ppl;
   return (in.P + tp - 1) / tp;
 }
 
@@ -185,6 +186,11 @@ b3h_status b3h_batch_create(const b3h_config *cfg, uint32_t count, const b3h_ins
   // measured 29.5 ms per batch against 33.9 ms with one (the measured choice
   // per workload is tuning.autotune_batch)
   b->PPL = cfg->lane_patterns == 1 ? 1u : 2u;
+  if (cfg->lane_states != 0 && cfg->lane_states != 1 && cfg->lane_states != 4) {
+    host_fail(B3H_ERR_ARG, "lane_states must be 0, 1 or 4");
+    return bail(B3H_ERR_ARG);
+  }
+  b->S1 = cfg->lane_states == 1;
   if (hipSetDevice(b->device) != hipSuccess ||
       hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     host_fail(B3H_ERR_HIP, "stream creation failed");
@@ -212,7 +218,7 @@ b3h_status b3h_batch_create(const b3h_config *cfg, uint32_t count, const b3h_ins
     in.pat_off = pat;
     pat += in.P;
     in.bsum_off = bsum;
-    bsum += (in.P + 64u * b->T - 1) / (64u * b->T);
+    bsum += b->S1 ? (in.P + 16u * b->T - 1) / (16u * b->T) : (in.P + 64u * b->T - 1) / (64u * b->T);
     in.pm_off = pm;
     pm += (size_t)in.I * 2 * K * 20;  // [entry][side][K][5][4]: columns, the fifth all ones
     const size_t slots = (size_t)(b->single ? 1 : 2) * in.I;
@@ -240,9 +246,13 @@ b3h_status b3h_batch_create(const b3h_config *cfg, uint32_t count, const b3h_ins
   b->st_bytes = st;
 
   // LDS stack capacity shared by all instances
+  auto lds_need = [&](uint32_t cap) {
+    const size_t four = prune_f32_lds_bytes(K, b->T, cap, max_I, 2);
+    return b->S1 ? std::max(four, prune_f32_s1_lds_bytes(K, b->T, cap, max_I)) : four;
+  };
   uint32_t cap = kMaxSlots;
-  while (cap > 1 && prune_f32_lds_bytes(K, b->T, cap, max_I, 2) > kMaxLds) cap--;
-  if (prune_f32_lds_bytes(K, b->T, cap, max_I, 2) > kMaxLds) {
+  while (cap > 1 && lds_need(cap) > kMaxLds) cap--;
+  if (lds_need(cap) > kMaxLds) {
     host_fail(B3H_ERR_UNSUPPORTED, "tree too large for the LDS-staged update list");
     return bail(B3H_ERR_UNSUPPORTED);
   }
@@ -400,6 +410,7 @@ b3h_status b3h_batch_propose_times(b3h_batch *b, const b3h_update *ups, uint32_t
   if (active == 0) return host_fail(B3H_ERR_ARG, "no instance has an update");
   const bool hbm = hbm_lists > 0;
   const uint32_t ppl = hbm ? 1u : b->PPL;
+  const bool s1 = b->S1 && !hbm;
   // commit: EditBuf flips, staging contents, arguments and the work list
   std::vector<uint32_t> order;
   for (uint32_t i = 0; i < b->count; i++) {
@@ -448,7 +459,7 @@ b3h_status b3h_batch_propose_times(b3h_batch *b, const b3h_update *ups, uint32_t
     a.T = b->T;
     a.nslots = std::max<uint32_t>(max_slots, 1u);
     a.cstride = in.cstride;
-    a.ntiles = tiles_for(b, in, ppl);
+    a.ntiles = tiles_for(b, in, ppl, s1);
   }
   // longest update lists first (their tiles take longest)
   std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
@@ -460,10 +471,12 @@ b3h_status b3h_batch_propose_times(b3h_batch *b, const b3h_update *ups, uint32_t
     const uint32_t nt = args[i].ntiles;
     for (uint32_t t = 0; t < nt; t++) items[nitems++] = make_uint2(i, t);
   }
-  const size_t lds = prune_f32_lds_bytes(K, b->T, std::max<uint32_t>(max_slots, 1u), max_n, ppl);
+  const size_t lds =
+      s1 ? prune_f32_s1_lds_bytes(K, b->T, std::max<uint32_t>(max_slots, 1u), max_n)
+         : prune_f32_lds_bytes(K, b->T, std::max<uint32_t>(max_slots, 1u), max_n, ppl);
   const uint32_t threads = 64u * K * b->T;
   uint32_t grid = 0;
-  BHIP(batch_grid(ppl, hbm, threads, lds, &grid));
+  BHIP(batch_grid(ppl, hbm, s1, threads, lds, &grid));
   grid = std::min(grid, nitems);
 
   const F32Args *d_args = reinterpret_cast<const F32Args *>(b->d_stage + b->st_args);
@@ -473,8 +486,8 @@ b3h_status b3h_batch_propose_times(b3h_batch *b, const b3h_update *ups, uint32_t
     BHIP(hipMemcpyAsync(b->d_stage, b->h_stage, b->st_bytes, hipMemcpyHostToDevice, b->stream));
     BHIP(launch_pbuild_batch(d_pd, b->count, max_n, K, 4, b->stream));
     if (e0) BHIP(hipEventRecord(e0, b->stream));
-    BHIP(launch_prune_f32_batch(d_args, d_items, nitems, b->queue, ppl, hbm, threads, lds, grid,
-                                b->stream));
+    BHIP(launch_prune_f32_batch(d_args, d_items, nitems, b->queue, ppl, hbm, s1, threads, lds,
+                                grid, b->stream));
     if (e1) BHIP(hipEventRecord(e1, b->stream));
     return B3H_OK;
   };
@@ -493,7 +506,7 @@ b3h_status b3h_batch_propose_times(b3h_batch *b, const b3h_update *ups, uint32_t
   }
   b3h_status st;
   if ((flags & B3H_BATCH_GRAPH) && !b->profiling) {
-    std::vector<uint64_t> key = {ppl, hbm, lds, grid, nitems, max_n};
+    std::vector<uint64_t> key = {ppl, hbm, s1, lds, grid, nitems, max_n};
     if (!b->graph || key != b->graph_key) {
       if (b->graph) {
         BHIP(hipGraphExecDestroy(b->graph));

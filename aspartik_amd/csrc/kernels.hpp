@@ -89,10 +89,12 @@ hipError_t launch_prune_f32_s1(const F32Args &a, uint32_t blocks, hipStream_t s)
 size_t prune_f32_s1_lds_bytes(uint32_t K, uint32_t T, uint32_t nslots, uint32_t n);
 // persistent batched form over many instances (one F32Args each)
 hipError_t prepare_prune_f32_batch(int max_lds_bytes);
-hipError_t batch_grid(uint32_t ppl, bool hbm, uint32_t threads, size_t lds, uint32_t *grid);
+// s1: one state per lane (prune_f32_s1_kernel's form; items of 16 T patterns)
+hipError_t batch_grid(uint32_t ppl, bool hbm, bool s1, uint32_t threads, size_t lds,
+                      uint32_t *grid);
 hipError_t launch_prune_f32_batch(const F32Args *insts, const uint2 *items, uint32_t nitems,
-                                  uint32_t *queue, uint32_t ppl, bool hbm, uint32_t threads,
-                                  size_t lds, uint32_t grid, hipStream_t s);
+                                  uint32_t *queue, uint32_t ppl, bool hbm, bool s1,
+                                  uint32_t threads, size_t lds, uint32_t grid, hipStream_t s);
 hipError_t prepare_prune_f32(int max_lds_bytes);
 // Partial updates that form one chain (entry 0 reads two stored children,
 // entry i > 0 the entry before it and one stored child): srcs[n + 1] lists

@@ -872,6 +872,40 @@ __global__ __launch_bounds__(512) void prune_f32_batch_kernel(const F32Args *ins
   }
 }
 
+// The persistent batched form with one state per lane (prune_tile_s1).
+__global__ __launch_bounds__(512) void prune_f32_batch_s1_kernel(const F32Args *insts,
+                                                                 const uint2 *items,
+                                                                 uint32_t nitems,
+                                                                 uint32_t *queue) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t item_s;
+  for (;;) {
+    if (threadIdx.x == 0) item_s = atomicAdd(&queue[0], 1u);
+    __syncthreads();
+    const uint32_t item = item_s;
+    __syncthreads();
+    if (item >= nitems) break;
+    const uint2 it = items[item];
+    const uint32_t inst = __builtin_amdgcn_readfirstlane(it.x);
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(it.y);
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef const __attribute__((address_space(4))) F32Args ConstArgs;
+    ConstArgs &a = ((ConstArgs *)insts)[inst];
+    prune_tile_s1(a, tile, smem);
+#else
+    prune_tile_s1(insts[inst], tile, smem);
+#endif
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+      __hip_atomic_store(&queue[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&queue[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 // Partial updates: update lists that form one chain -- entry 0 reads two
 // stored children (leaves or partials of an earlier evaluation), every later
 // entry reads the entry just before it and one stored child (the path from
@@ -1100,7 +1134,8 @@ hipError_t prepare_prune_f32(int max_lds_bytes) {
 hipError_t prepare_prune_f32_batch(int max_lds_bytes) {
   const void *fns[] = {reinterpret_cast<const void *>(&prune_f32_batch_kernel<1, false>),
                        reinterpret_cast<const void *>(&prune_f32_batch_kernel<1, true>),
-                       reinterpret_cast<const void *>(&prune_f32_batch_kernel<2, false>)};
+                       reinterpret_cast<const void *>(&prune_f32_batch_kernel<2, false>),
+                       reinterpret_cast<const void *>(&prune_f32_batch_s1_kernel)};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        max_lds_bytes);
@@ -1109,8 +1144,10 @@ hipError_t prepare_prune_f32_batch(int max_lds_bytes) {
   return hipSuccess;
 }
 
-hipError_t batch_grid(uint32_t ppl, bool hbm, uint32_t threads, size_t lds, uint32_t *grid) {
-  const void *f = (ppl == 2 && !hbm)
+hipError_t batch_grid(uint32_t ppl, bool hbm, bool s1, uint32_t threads, size_t lds,
+                      uint32_t *grid) {
+  const void *f = s1 ? reinterpret_cast<const void *>(&prune_f32_batch_s1_kernel)
+                  : (ppl == 2 && !hbm)
                       ? reinterpret_cast<const void *>(&prune_f32_batch_kernel<2, false>)
                       : hbm ? reinterpret_cast<const void *>(&prune_f32_batch_kernel<1, true>)
                             : reinterpret_cast<const void *>(&prune_f32_batch_kernel<1, false>);
@@ -1126,11 +1163,13 @@ hipError_t batch_grid(uint32_t ppl, bool hbm, uint32_t threads, size_t lds, uint
 }
 
 hipError_t launch_prune_f32_batch(const F32Args *insts, const uint2 *items, uint32_t nitems,
-                                  uint32_t *queue, uint32_t ppl, bool hbm, uint32_t threads,
-                                  size_t lds, uint32_t grid, hipStream_t s) {
+                                  uint32_t *queue, uint32_t ppl, bool hbm, bool s1,
+                                  uint32_t threads, size_t lds, uint32_t grid, hipStream_t s) {
   const dim3 g(grid), b(threads);
   const uint32_t l = (uint32_t)lds;
-  if (ppl == 2 && !hbm)
+  if (s1)
+    hipLaunchKernelGGL(prune_f32_batch_s1_kernel, g, b, l, s, insts, items, nitems, queue);
+  else if (ppl == 2 && !hbm)
     hipLaunchKernelGGL((prune_f32_batch_kernel<2, false>), g, b, l, s, insts, items, nitems, queue);
   else if (hbm)
     hipLaunchKernelGGL((prune_f32_batch_kernel<1, true>), g, b, l, s, insts, items, nitems, queue);

@@ -194,15 +194,17 @@ def bench_batch(args):
     data = [(x["codes"], None) for x in insts]
     models = [(x["model"].q_matrix(), x["model"].frequencies(), True) for x in insts]
     if args.batch_tile:
-        tile, lanes = (int(v) for v in args.batch_tile.split("x"))
+        parts = [int(v) for v in args.batch_tile.split("x")]
+        tile, lanes, states = (parts + [4])[:3]
         trials = []
     else:
         log("[bench] batched: tuning tiles")
-        (tile, lanes), trials = autotune_batch(data, models, ups, K, evals=6)
+        (tile, lanes, states), trials = autotune_batch(data, models, ups, K, evals=6)
     for tr in trials:
-        log(f"[bench]   tile {tr.tile_patterns} x {tr.lane_patterns}/lane: {tr.ms:.2f} ms per "
-            f"batch, kernel {tr.kernel_ms:.2f} ms")
-    b = Batch(data, num_categories=K, single_slot=True, tile_patterns=tile, lane_patterns=lanes)
+        log(f"[bench]   tile {tr.tile_patterns} x {tr.lane_patterns} patterns/lane x "
+            f"{tr.lane_states} states/lane: {tr.ms:.2f} ms per batch, kernel {tr.kernel_ms:.2f} ms")
+    b = Batch(data, num_categories=K, single_slot=True, tile_patterns=tile, lane_patterns=lanes,
+              lane_states=states)
     for i, (q, f, special) in enumerate(models):
         b.set_model(i, q, f, special)
     log("[bench] batched: warmup")
@@ -301,7 +303,7 @@ def bench_batch(args):
         "config": {"workload": f"hky_gamma4_batch{args.instances}", "instances": args.instances,
                    "taxa": "U{50..500}", "patterns": "U{2000..50000}", "categories": K,
                    "evaluation": "full, all instances per step, graph replay, single slot",
-                   "tile_patterns": tile, "lane_patterns": lanes},
+                   "tile_patterns": tile, "lane_patterns": lanes, "lane_states": states},
         "roofline": {"bound": "hbm", "achieved": B / (kernel_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": B / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -317,7 +319,8 @@ def bench_batch(args):
                                      f"{f64_evals} timed full evaluations after one untimed"},
         "device_bytes": device_bytes,
         "tile_trials": [{"tile_patterns": tr.tile_patterns, "lane_patterns": tr.lane_patterns,
-                         "ms_per_batch": tr.ms, "kernel_ms": tr.kernel_ms} for tr in trials],
+                         "lane_states": tr.lane_states, "ms_per_batch": tr.ms,
+                         "kernel_ms": tr.kernel_ms} for tr in trials],
     }), flush=True)
 
 
@@ -930,8 +933,8 @@ def main():
     ap.add_argument("--codon-patterns", type=int, default=20_000)
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--batch-tile", default="",
-                    help="batched workload: tile shape PxL (patterns per workgroup x per lane) "
-                         "instead of the autotuner's choice")
+                    help="batched workload: tile shape PxL or PxLxS (tile_patterns x patterns "
+                         "per lane x states per lane) instead of the autotuner's choice")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

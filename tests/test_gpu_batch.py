@@ -172,12 +172,14 @@ def test_batch_autotune_picks_a_candidate_with_identical_sites(hip_device):
     data = [(x["codes"], x["w"]) for x in insts]
     models = [(x["model"].q_matrix(), x["model"].frequencies(), True) for x in insts]
     ups = [_full_update(x, rates) for x in insts]
-    (tile, lanes), trials = autotune_batch(data, models, ups, K, evals=2)
-    assert (tile, lanes) in batch_candidates(K)
+    (tile, lanes, states), trials = autotune_batch(data, models, ups, K, evals=2)
+    assert (tile, lanes, states) in batch_candidates(K)
     assert len(trials) == len(batch_candidates(K)) and trials[0].ms <= trials[-1].ms
     results = []
-    for tp, lp in [(0, 0), (tile, lanes)]:
-        b = Batch(data, num_categories=K, single_slot=True, tile_patterns=tp, lane_patterns=lp)
+    # the default shape, the tuner's pick and one state per lane
+    for tp, lp, ls in [(0, 0, 0), (tile, lanes, states), (128, 1, 1)]:
+        b = Batch(data, num_categories=K, single_slot=True, tile_patterns=tp, lane_patterns=lp,
+                  lane_states=ls)
         for i, (q, f, sc) in enumerate(models):
             b.set_model(i, q, f, sc)
         b.propose_times(ups)
@@ -185,6 +187,7 @@ def test_batch_autotune_picks_a_candidate_with_identical_sites(hip_device):
         totals = b.totals()
         b.close()
         assert np.all(np.isfinite(totals))
-    for a, c in zip(*results):
-        for x, y in zip(a, c):
-            np.testing.assert_array_equal(x, y)
+    for other in results[1:]:
+        for a, c in zip(results[0], other):
+            for x, y in zip(a, c):
+                np.testing.assert_array_equal(x, y)
