@@ -495,6 +495,9 @@ def bench_mcmc(args):
     block = 100
     if thin > 1:
         collect = min(collect, args.burnin // thin) // block * block
+        if collect == 0:
+            raise SystemExit("--collect-every: the burn-in is too short for one block of 100 "
+                             "collected steps")
     span = collect * thin  # the burn-in steps the training pairs come from
     if span < args.burnin:   # training pairs from the last `span` steps only
         burn.run(args.burnin - span)
