@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -568,7 +569,8 @@ def bench_mcmc(args):
         res[name] = dict(stats=st, ess=e, ess_batch_means=ebm, min_ess=min(e.values()),
                          ess_per_s=min(e.values()) / st["seconds"],
                          ess_per_s_by_statistic={k: v / st["seconds"] for k, v in e.items()},
-                         means={k: float(trace[:, col].mean()) for k, col in cols})
+                         means={k: float(trace[:, col].mean()) for k, col in cols},
+                         sds={k: float(trace[:, col].std()) for k, col in cols})
         log(f"[bench] {name}: {st['seconds']:.1f} s (likelihood {st['likelihood_seconds']:.1f} s, "
             f"surrogate {st['surrogate_seconds']:.1f} s), accepted {st['op_accepted']} of "
             f"{st['op_proposed']}, exact {st['exact_evals']}, ESS {e}")
@@ -580,6 +582,15 @@ def bench_mcmc(args):
                 (res["plain"]["ess_batch_means"][k] / res["plain"]["stats"]["seconds"])
                 for k in res["plain"]["ess_batch_means"]}
     log(f"[bench] ESS/s ratios (Geyer) {ratio_by_stat}; batch means {ratio_bm}")
+    # posterior summaries: each statistic's mean, delayed minus plain, in
+    # units of the two chains' combined Monte Carlo standard error (ESS-based)
+    agree = {}
+    pr, dr = res["plain"], res["delayed"]
+    for k in pr["means"]:
+        se = math.sqrt(pr["sds"][k] ** 2 / max(pr["ess"][k], 1.0) +
+                       dr["sds"][k] ** 2 / max(dr["ess"][k], 1.0))
+        agree[k] = (dr["means"][k] - pr["means"][k]) / se if se > 0 else 0.0
+    log(f"[bench] posterior means, delayed - plain in combined MCSE units: {agree}")
     ratio_incl_training = (res["delayed"]["min_ess"] /
                            (res["delayed"]["stats"]["seconds"] + t_train)) / res["plain"]["ess_per_s"]
     cpu = None
@@ -635,6 +646,7 @@ def bench_mcmc(args):
         "ess_ratio_delayed_vs_plain": ratio,
         "ess_ratio_by_statistic": ratio_by_stat,
         "ess_ratio_by_statistic_batch_means": ratio_bm,
+        "posterior_agreement_z": agree,
         "ess_ratio_including_training": ratio_incl_training,
         "plain": res["plain"], "delayed": res["delayed"], "surrogate_held_out": held,
         "other_betas": {n: dict(v, ess_ratio_vs_plain=v["ess_per_s"] / res["plain"]["ess_per_s"])

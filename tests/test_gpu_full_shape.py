@@ -1,7 +1,8 @@
 """Parity at the full sizes of the measured workloads, against the FP64 oracle.
 
 - GTR+Gamma4, 128 taxa, 100,000 unique simulated patterns with weights
-  U{1..5}, seed 4 (the benchmark workload): a full evaluation and one-edge
+  U{1..5}, seed 4 (the benchmark workload), and the delayed-acceptance
+  benchmark's 200 taxa x 200,000 sites: a full evaluation and one-edge
   partial evaluations after node-height moves, per site and in total.
 - Batched HKY+Gamma4: 256 instances of 50-500 taxa and 2,000-50,000
   patterns, evaluated in one launch (single slot, as benchmarked); every
@@ -50,8 +51,17 @@ def _check_sites(got, ref):
 
 
 def test_gtr_gamma4_128taxa_100000pat(hip_device):
-    wl = workloads.gtr_gamma4(128, 100_000, seed=4)
-    L, P, K = 128, 100_000, wl["K"]
+    _gtr_gamma4_full_shape(workloads.gtr_gamma4(128, 100_000, seed=4), 128, 100_000)
+
+
+def test_gtr_gamma4_200taxa_200000sites_mcmc_workload(hip_device):
+    """The delayed-acceptance benchmark's likelihood: 200 taxa, 200,000
+    simulated sites (weights 1)."""
+    _gtr_gamma4_full_shape(workloads.gtr_gamma4_mcmc(200, 200_000), 200, 200_000)
+
+
+def _gtr_gamma4_full_shape(wl, L, P):
+    K = wl["K"]
     print(f"[full-shape] GTR+Gamma4 {L} taxa {P} patterns", flush=True)
     tree = DatedTree(L, wl["parents"], wl["children"], wl["heights"])
     model = wl["model"]
