@@ -14,7 +14,8 @@
 //    so one partial is one float4 (16 B) and a wavefront moves PPL KiB per
 //    node in coalesced 1 KiB rows; PPL independent chains per lane hide
 //    latency and halve the wavefront count so a whole evaluation fits on the
-//    chip at once.
+//    chip at once.  (prune_tile_s1 is the one-state-per-lane form: a quad of
+//    lanes per pattern, 16 patterns per wavefront, bit-identical values.)
 //  - partials [slot][K][P] float4 with an int16 cumulative binary exponent per
 //    (slot, category, pattern).  After every node the four values are scaled
 //    by 2^-e (e = exponent of their maximum), which is exact, so FP32 never
