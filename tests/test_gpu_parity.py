@@ -268,7 +268,7 @@ def test_one_state_per_lane_is_bit_identical(hip_device, L, P, K, tile):
     DPP) computes the same site values as four states per lane, bit for bit:
     in full evaluations, in the partial updates that follow (which read the
     partials it stored, through the four-states kernels), with per-pattern
-    classes, and with the in-place reads of stored children that a deep
+    classes and per-category models and branch lengths, and with the in-place reads of stored children that a deep
     update list needs once the LDS stack is full (400 taxa).  Totals agree to
     the rounding of the weighted sum's grouping (16 patterns per wavefront
     instead of 64 or 128)."""
@@ -320,6 +320,20 @@ def test_one_state_per_lane_is_bit_identical(hip_device, L, P, K, tile):
         t2 = tree.copy()
         t2.mark_all_edges_updated()
         evaluate(c, t2)
+    same()
+    # per-category models and branch lengths (each category its own root
+    # frequencies)
+    subs = [GTR(rng.dirichlet([20.0] * 4), rng.dirichlet([4.0] * 6)) for _ in range(K)]
+    t2 = tree.copy()
+    t2.mark_all_edges_updated()
+    internals, children = t2.partials_lists()
+    flat = children.reshape(-1).astype(np.int64)
+    times = t2.heights[t2.parents[flat]] - t2.heights[flat]
+    lengths = np.stack([times * rng.uniform(0.5, 2.0, size=times.size) for _ in range(K)])
+    for c in (base, one):
+        for k, sub in enumerate(subs):
+            c.set_category_model(k, sub.q_matrix(), sub.frequencies())
+        c.propose_lengths(internals, children, lengths)
     same()
     base.close()
     one.close()
