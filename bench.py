@@ -348,6 +348,29 @@ def bench_codon(args):
     log(f"[bench] codon full: {args.steps / dt:.1f} evals/s, kernel median {kernel_ms * 1e3:.1f} us")
     site_gpu = np.array([calc.likelihoods(k) for k in range(K)])
     mixed_gpu = calc.site_mixed()
+    # one-edge partial evaluations: one random non-root edge and its path to
+    # the root (propose, read the total, accept), then the full state back
+    rng = np.random.default_rng(99)
+    edges = [int(e) for e in rng.choice(2 * L - 2, size=32) if int(e) != tree.root]
+    plists = [workloads.path_lists(tree, e) for e in edges]
+    ptimes = [workloads.edge_times(tree, pl[1]) for pl in plists]
+    for i in range(50):
+        calc.propose_times(*plists[i % len(plists)], ptimes[i % len(plists)], rates)
+        calc.total()
+        calc.accept()
+    nparts = max(500, args.steps)
+    t0 = time.perf_counter()
+    for i in range(nparts):
+        j = i % len(plists)
+        calc.propose_times(*plists[j], ptimes[j], rates)
+        calc.total()
+        calc.accept()
+    dt_part = time.perf_counter() - t0
+    mean_path = float(np.mean([pl[0].size for pl in plists]))
+    log(f"[bench] codon partial: {nparts / dt_part:.1f} evals/s (mean path {mean_path:.1f} nodes)")
+    calc.propose_times(*full, times_full, rates)
+    calc.total()
+    calc.accept()
     dev_bytes = calc.device_bytes()
     calc.close()
     root = tree.root
@@ -413,6 +436,7 @@ def bench_codon(args):
         "algorithmic_bytes_per_eval": B, "reference_algorithm_bytes_per_eval": B_ref,
         "algorithmic_flops_per_eval": F, "error_vs_fp64_oracle": err,
         "device_bytes": dev_bytes,
+        "partial_evals_per_s": nparts / dt_part, "partial_mean_path_nodes": mean_path,
     }), flush=True)
 
 
